@@ -1,0 +1,83 @@
+// Subband image history ("ring"): time-major rows per (subband, pixel).
+//
+// ring_new[s][p][0:hist)            <- ring_old[s][p][prev_len : prev_len + hist)
+// ring_new[s][p][hist : hist + n)   <- img[s][0:n][p]   (64 x 64 LDS transpose)
+// Old and new buffers alternate, so the history move never overlaps itself.
+#include "dsx_internal.h"
+
+namespace dsx {
+
+template <typename T>
+__global__ __launch_bounds__(256) void ring_hist_kernel(const T *old_ring, T *new_ring, int64_t n_rows,
+                                                        int hist, int row_len, int prev_len) {
+    const int64_t n = n_rows * hist;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / hist;
+        const int k = (int)(i - row * hist);
+        new_ring[row * row_len + k] = old_ring[row * row_len + prev_len + k];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void ring_transpose_kernel(const T *img, T *ring, int n_sub,
+                                                             int n_pix, int n_new, int hist,
+                                                             int row_len) {
+    __shared__ T tile[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 4
+    const int tiles_p = (n_pix + 63) / 64, tiles_t = (n_new + 63) / 64;
+    const int64_t n_tiles = (int64_t)n_sub * tiles_p * tiles_t;
+    for (int64_t b = blockIdx.x; b < n_tiles; b += gridDim.x) {
+        const int s = (int)(b / ((int64_t)tiles_p * tiles_t));
+        const int rem = (int)(b - (int64_t)s * tiles_p * tiles_t);
+        const int tt = rem / tiles_p, tp = rem - tt * tiles_p;
+        const int t0 = tt * 64, p0 = tp * 64;
+        __syncthreads();
+        for (int r = ty; r < 64; r += 4) {
+            const int t = t0 + r, p = p0 + tx;
+            if (t < n_new && p < n_pix) tile[r][tx] = img[((int64_t)s * n_new + t) * n_pix + p];
+        }
+        __syncthreads();
+        for (int r = ty; r < 64; r += 4) {
+            const int p = p0 + r, t = t0 + tx;
+            if (t < n_new && p < n_pix)
+                ring[((int64_t)s * n_pix + p) * row_len + hist + t] = tile[tx][r];
+        }
+    }
+}
+
+}  // namespace dsx
+
+using namespace dsx;
+
+extern "C" int dsx_ring_advance(const void *ring_old, void *ring_new, const void *img, int n_sub,
+                                int n_pix, int hist, int cap, int prev_len, int n_new, int dtype,
+                                dsx_stream_t stream) {
+    DSX_REQUIRE(ring_old && ring_new && img, "dsx_ring_advance: null pointer");
+    DSX_REQUIRE(ring_old != ring_new, "dsx_ring_advance: old and new ring must differ");
+    DSX_REQUIRE(n_sub > 0 && n_pix > 0 && hist >= 0 && cap > 0, "dsx_ring_advance: bad shape");
+    DSX_REQUIRE(prev_len >= 0 && prev_len <= cap, "dsx_ring_advance: prev_len out of range");
+    DSX_REQUIRE(n_new > 0 && n_new <= cap, "dsx_ring_advance: n_new out of range");
+    DSX_REQUIRE(dtype == DSX_F16 || dtype == DSX_F32, "dsx_ring_advance: dtype");
+    const int row_len = hist + cap;
+    const int64_t n_rows = (int64_t)n_sub * n_pix;
+    hipStream_t s = as_stream(stream);
+    const int64_t nh = n_rows * hist;
+    const int hb = (int)std::min<int64_t>((nh + 255) / 256, 65536);
+    const int64_t n_tiles = (int64_t)n_sub * ((n_pix + 63) / 64) * ((n_new + 63) / 64);
+    const int tb = (int)std::min<int64_t>(n_tiles, 65536);
+    if (dtype == DSX_F16) {
+        if (nh > 0)
+            ring_hist_kernel<_Float16><<<hb, 256, 0, s>>>((const _Float16 *)ring_old, (_Float16 *)ring_new,
+                                                           n_rows, hist, row_len, prev_len);
+        ring_transpose_kernel<_Float16><<<tb, 256, 0, s>>>((const _Float16 *)img, (_Float16 *)ring_new,
+                                                            n_sub, n_pix, n_new, hist, row_len);
+    } else {
+        if (nh > 0)
+            ring_hist_kernel<float><<<hb, 256, 0, s>>>((const float *)ring_old, (float *)ring_new,
+                                                        n_rows, hist, row_len, prev_len);
+        ring_transpose_kernel<float><<<tb, 256, 0, s>>>((const float *)img, (float *)ring_new, n_sub,
+                                                         n_pix, n_new, hist, row_len);
+    }
+    return check_launch("dsx_ring_advance");
+}

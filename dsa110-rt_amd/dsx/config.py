@@ -1,0 +1,192 @@
+"""Node configuration: array, band, block, grid, DM plan and detector knobs.
+
+The reference hard-wires one op-point (96 antennas, 384 channels,
+``src/dsart/common/constants.py:53-208``); the fast path here takes every
+shape from a :class:`NodeConfig` so the same kernels serve the 16-antenna
+CPU-sized case and the 96-antenna node.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field, replace
+from typing import Optional, Tuple
+
+import numpy as np
+
+from .constants import (
+    DELTA_NU_CH_GHZ,
+    NATIVE_SAMPLE_US,
+    NTIMES_PER_PACKET,
+    freq_GHz,
+    nbase,
+)
+from .dispersion import subband_shift_table
+
+#: Dedispersed centres trail the newest imaged step by this many steps, and
+#: candidate maxima are taken over aligned chunks of the same length.
+DETECT_DELAY_STEPS = 64
+DETECT_CHUNK_STEPS = 64
+
+
+@dataclass(frozen=True)
+class NodeConfig:
+    """Everything the fast path needs to know about one node stream."""
+
+    name: str
+    n_ant: int
+    chan_freqs_GHz: Tuple[float, ...]      # (n_chan,) descending
+    n_samp: int                            # native samples per full block
+    t_int: int = 8                         # native samples per step
+    csf: int = 1                           # channels summed inside the correlator
+    n_sub: int = 1                         # uv planes (subbands) per step
+    n_grid: int = 256
+    grid_freq_GHz: Optional[float] = None  # None: each output channel's own frequency
+    cell_lambda: Optional[float] = None    # None: auto-fit to the longest baseline
+    dm_max: float = 500.0
+    n_dm: int = 64
+    widths: Tuple[int, ...] = (1, 2, 4, 8)
+    threshold: float = 8.0
+    min_stat_count: int = 512              # complete samples per DM before it is scored
+    n_top: int = 256
+    radius_t: int = 128
+    radius_dm: int = 5
+    radius_lm: int = 3
+    kernel_support: int = 1
+    array_ew_m: float = 1500.0
+    array_ns_m: float = 400.0
+    layout_seed: int = 0xD5A1107E
+    volt_seed: int = 0xD5A1107F
+    sigma_volt: float = 2.5
+
+    # ---- derived -------------------------------------------------------
+    @property
+    def n_chan(self) -> int:
+        return len(self.chan_freqs_GHz)
+
+    @property
+    def n_pkt(self) -> int:
+        return self.n_samp // NTIMES_PER_PACKET
+
+    @property
+    def n_steps(self) -> int:
+        return self.n_samp // self.t_int
+
+    @property
+    def nbase(self) -> int:
+        return nbase(self.n_ant)
+
+    @property
+    def n_chan_out(self) -> int:
+        return self.n_chan // self.csf
+
+    @property
+    def per_sub(self) -> int:
+        return self.n_chan_out // self.n_sub
+
+    @property
+    def t_step_us(self) -> float:
+        return self.t_int * NATIVE_SAMPLE_US
+
+    @property
+    def block_seconds(self) -> float:
+        return self.n_samp * NATIVE_SAMPLE_US * 1e-6
+
+    @property
+    def freq_out_GHz(self) -> np.ndarray:
+        """Band-centre frequency of each correlator output channel.
+
+        Mean of its ``csf`` fine channels (sparsity_pattern.py:1065-1073).
+        """
+        f = np.asarray(self.chan_freqs_GHz, dtype=np.float64)
+        if self.csf == 1:
+            return f
+        return f.reshape(self.n_chan_out, self.csf).mean(axis=1)
+
+    @property
+    def grid_freq_out_GHz(self) -> np.ndarray:
+        """Frequency used to scale baselines into wavelengths per output channel."""
+        if self.grid_freq_GHz is None:
+            return self.freq_out_GHz
+        return np.full(self.n_chan_out, float(self.grid_freq_GHz), dtype=np.float64)
+
+    @property
+    def subband_ref_GHz(self) -> np.ndarray:
+        """Reference frequency of each subband: its first (highest) output channel.
+
+        Same rule as ``coarse_dm/dm_plan.py:179-191``.
+        """
+        return np.ascontiguousarray(self.freq_out_GHz[:: self.per_sub][: self.n_sub])
+
+    @property
+    def nu_bot_GHz(self) -> float:
+        return float(self.chan_freqs_GHz[-1])
+
+    @property
+    def dm_trials(self) -> np.ndarray:
+        return np.linspace(0.0, float(self.dm_max), int(self.n_dm))
+
+    def shift_table(self) -> np.ndarray:
+        return subband_shift_table(self.dm_trials, self.subband_ref_GHz,
+                                   self.nu_bot_GHz, self.t_step_us)
+
+    @property
+    def w_max(self) -> int:
+        return int(max(self.widths))
+
+    def validate(self, detect: bool = True) -> None:
+        """Check shapes; ``detect`` adds the block/width rules of the detector."""
+        if self.n_ant < 2:
+            raise ValueError("n_ant must be >= 2")
+        if self.n_samp % (NTIMES_PER_PACKET * self.t_int) != 0:
+            raise ValueError("n_samp must be a multiple of 2 * t_int")
+        if self.t_int % NTIMES_PER_PACKET != 0:
+            raise ValueError("t_int must be a multiple of 2")
+        if self.n_chan % self.csf != 0:
+            raise ValueError("csf must divide n_chan")
+        if self.n_chan_out % self.n_sub != 0:
+            raise ValueError("n_sub must divide n_chan / csf")
+        if self.n_grid & (self.n_grid - 1) or not 8 <= self.n_grid <= 1024:
+            raise ValueError("n_grid must be a power of two in [8, 1024]")
+        if self.kernel_support != 1:
+            raise ValueError("only the nearest-cell gridder (kernel_support=1) is supported")
+        if not detect:
+            return
+        if self.n_steps % DETECT_CHUNK_STEPS != 0:
+            raise ValueError(f"steps per block must be a multiple of {DETECT_CHUNK_STEPS}")
+        ws = list(self.widths)
+        if ws != sorted(set(ws)) or ws[0] != 1 or any(w & (w - 1) for w in ws):
+            raise ValueError("widths must be ascending powers of two starting at 1")
+        if ws != [1 << k for k in range(len(ws))]:
+            raise ValueError("widths must be 1, 2, 4, ... without gaps")
+        if self.w_max > 2 * DETECT_DELAY_STEPS:
+            raise ValueError("widest boxcar exceeds twice the detection delay")
+
+    def with_(self, **kw) -> "NodeConfig":
+        return replace(self, **kw)
+
+
+def c1_config() -> NodeConfig:
+    """16 ant x 64 ch (2.93 MHz, 1498.75 MHz down) x 2048 samples, 64 DMs 0-500."""
+    dnu = 96 * DELTA_NU_CH_GHZ
+    top = freq_GHz(0, 0)
+    freqs = tuple(top - c * dnu for c in range(64))
+    return NodeConfig(
+        name="c1_16ant_64ch", n_ant=16, chan_freqs_GHz=freqs, n_samp=2048, t_int=8,
+        csf=1, n_sub=64, n_grid=64, grid_freq_GHz=None,
+        dm_max=500.0, n_dm=64, widths=(1, 2, 4, 8), threshold=8.0,
+    )
+
+
+def c2_config() -> NodeConfig:
+    """96 ant x 384 ch (chgroup 0) x 16384 samples; 16 subbands, 256^2 grid, 256 DMs."""
+    freqs = tuple(freq_GHz(0, c) for c in range(384))
+    return NodeConfig(
+        name="c2_96ant_384ch", n_ant=96, chan_freqs_GHz=freqs, n_samp=16384, t_int=8,
+        csf=24, n_sub=16, n_grid=256, grid_freq_GHz=1.4,
+        dm_max=1500.0, n_dm=256, widths=(1, 2, 4, 8, 16, 32), threshold=8.0,
+        layout_seed=0xD5A11096, volt_seed=0xD5A11097,
+    )
+
+
+def c2_small_config(n_samp: int = 1024) -> NodeConfig:
+    """The 96-antenna node at a short block, for parity tests."""
+    return c2_config().with_(name=f"c2_96ant_384ch_{n_samp}", n_samp=n_samp)

@@ -1,0 +1,200 @@
+"""Single-GPU search node: raw 4-bit voltage blocks in, candidate lists out.
+
+Per block (all on one HIP stream, no host round trip until the candidate
+copy):
+
+1. ``dsx_corr_i8``       raw bytes -> exact int32 visibilities (step, channel, bls)
+2. ``dsx_image_subbands`` visibilities -> masked dirty images (subband, step, N, N)
+3. ``dsx_ring_advance``   images -> time-major subband history (double buffered)
+4. ``dsx_dedisp_detect``  history -> chunk-maximum candidates + float64 stats
+5. host decode           top-k + merge radius -> ``List[Candidate]``
+
+This is the caller the reference spreads over ``corr_fast_integration.
+process_block`` (``src/dsart/services/corr_fast_integration.py:2063-2585``)
+and ``CubePipeline.process`` (``src/dsart/services/cube_pipeline.py:2641-2728``),
+collapsed onto one GPU with no transport in between.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import DETECT_CHUNK_STEPS, DETECT_DELAY_STEPS, NodeConfig
+from .detector import CAND_DTYPE, Candidate, decode_chunk_maxima
+from .imager import mask_npad
+from .pattern import build_cell_map, build_imager_taps
+
+LDS_BUDGET_DEDISP = 64 * 1024
+
+
+@dataclass
+class BlockResult:
+    candidates: List[Candidate]
+    n_raw: int                 # chunk maxima above threshold (before decode)
+    overflow: bool
+    T0: int
+    n_steps: int
+    records: Optional[np.ndarray] = None   # raw chunk-maximum records (CAND_DTYPE)
+
+
+class SearchNode:
+    """Stateful node pipeline; state (history ring, statistics) stays on the GPU."""
+
+    def __init__(self, cfg: NodeConfig, antpos_e: np.ndarray, antpos_n: np.ndarray, *,
+                 device: str = "cuda", ring_dtype: torch.dtype = torch.float16,
+                 max_block_steps: Optional[int] = None, cand_cap: int = 1 << 20,
+                 stream: Optional[torch.cuda.Stream] = None):
+        cfg.validate()
+        if ring_dtype not in (torch.float16, torch.float32):
+            raise TypeError("ring_dtype must be float16 or float32")
+        if not torch.cuda.is_available():
+            raise RuntimeError("SearchNode needs a ROCm GPU")
+        _lib.lib()
+        self.cfg = cfg
+        self.dev = torch.device(device)
+        self.ring_dtype = ring_dtype
+        self.stream = stream
+        N = cfg.n_grid
+        self.n_pix = N * N
+        # ---- gridder taps
+        self.cmap = build_cell_map(antpos_e, antpos_n, cfg.grid_freq_out_GHz, N,
+                                   cell_lambda=cfg.cell_lambda)
+        taps = build_imager_taps(self.cmap, cfg.n_sub)
+        self.n_rows = taps.n_rows
+        self.tap_src = torch.from_numpy(taps.src).to(self.dev)
+        self.tap_dst = torch.from_numpy(taps.dst).to(self.dev)
+        self.tap_off = torch.from_numpy(taps.offsets).to(self.dev)
+        self.npad = mask_npad(N, cfg.kernel_support)
+        # ---- DM plan
+        sh = cfg.shift_table()                                   # [d][s]
+        self.shifts_host = sh
+        self.shift = torch.from_numpy(np.ascontiguousarray(sh.T)).to(self.dev)   # [s][d]
+        self.valid_from_host = sh.max(axis=1).astype(np.int64)
+        self.valid_from = torch.from_numpy(self.valid_from_host.astype(np.int32)).to(self.dev)
+        self.shmax = sh.max(axis=0).astype(np.int32)             # per subband
+        self.n_widths = len(cfg.widths)
+        self.w_max = cfg.w_max
+        h2 = self.w_max // 2
+        self.hist = DETECT_DELAY_STEPS + h2 + int(sh.max()) + 1
+        self.cap = int(max_block_steps or cfg.n_steps)
+        # ---- buffers
+        nb = cfg.nbase
+        self.vis = torch.empty((self.cap, cfg.n_chan_out, nb, 2), dtype=torch.int32, device=self.dev)
+        self.img = torch.empty((cfg.n_sub, self.cap, N, N), dtype=ring_dtype, device=self.dev)
+        row_len = self.hist + self.cap
+        self.row_len = row_len
+        self.rings = [torch.zeros((cfg.n_sub, self.n_pix, row_len), dtype=ring_dtype, device=self.dev)
+                      for _ in range(2)]
+        self.cur = 0
+        self.mean = torch.zeros((self.n_pix, cfg.n_dm), dtype=torch.float64, device=self.dev)
+        self.m2 = torch.zeros((self.n_pix, cfg.n_dm), dtype=torch.float64, device=self.dev)
+        self.n_prev_host = np.zeros(cfg.n_dm, dtype=np.float64)
+        self.n_prev = torch.zeros(cfg.n_dm, dtype=torch.float64, device=self.dev)
+        self.blk_count = torch.zeros(cfg.n_dm, dtype=torch.float64, device=self.dev)
+        self.cand_cap = int(cand_cap)
+        self.cand = torch.empty((self.cand_cap, 4), dtype=torch.int32, device=self.dev)
+        self.n_cand = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        self.tiles: Dict[int, tuple] = {}
+        self.T0 = 0
+        self.prev_len = 0
+
+    # ------------------------------------------------------------------ helpers
+    def _tile_for(self, n_centres: int):
+        if n_centres not in self.tiles:
+            shmax = np.ascontiguousarray(self.shmax, dtype=np.int32)
+            tile = _lib.call("dsx_dedisp_tile", self.cfg.n_sub, shmax.ctypes.data_as(_lib._P),
+                             self.n_widths, n_centres, LDS_BUDGET_DEDISP)
+            if tile <= 0:
+                tile = _lib.call("dsx_dedisp_tile", self.cfg.n_sub, shmax.ctypes.data_as(_lib._P),
+                                 self.n_widths, n_centres, 160 * 1024)
+            if tile <= 0:
+                raise RuntimeError("DM plan too wide for the LDS windows of dsx_dedisp_detect")
+            ww = 2 * (self.w_max // 2)
+            lens = tile + ww + self.shmax.astype(np.int64)
+            offs = np.concatenate([[0], np.cumsum(lens)[:-1]])
+            info = np.stack([offs, self.shmax], axis=1).astype(np.int32)
+            self.tiles[n_centres] = (tile, int(lens.sum()),
+                                     torch.from_numpy(np.ascontiguousarray(info)).to(self.dev))
+        return self.tiles[n_centres]
+
+    def _block_counts(self, c_lo: int, L: int) -> np.ndarray:
+        start = np.maximum(self.valid_from_host, c_lo)
+        return np.clip(c_lo + L - start, 0, L).astype(np.float64)
+
+    # ------------------------------------------------------------------ stages
+    def correlate(self, raw: torch.Tensor, n_samp: int) -> torch.Tensor:
+        cfg = self.cfg
+        n_pkt = n_samp // 2
+        n_steps = n_samp // cfg.t_int
+        _lib.require_gpu(raw)
+        if raw.dtype != torch.uint8 or raw.numel() != n_pkt * cfg.n_ant * cfg.n_chan * 4:
+            raise ValueError("raw must be uint8 of shape (n_pkt, n_ant, n_chan, 2, 2)")
+        vis = self.vis[:n_steps]
+        _lib.call("dsx_corr_i8", _lib.ptr(raw), None, _lib.ptr(vis), n_pkt, cfg.n_ant, cfg.n_chan,
+                  cfg.t_int, cfg.csf, 1, 0, 1, _lib.stream(self.stream))
+        return vis
+
+    def image(self, vis: torch.Tensor, n_steps: int) -> torch.Tensor:
+        cfg = self.cfg
+        img = self.img[:, :n_steps] if n_steps == self.cap else \
+            torch.empty((cfg.n_sub, n_steps, cfg.n_grid, cfg.n_grid), dtype=self.ring_dtype, device=self.dev)
+        _lib.call("dsx_image_subbands", _lib.ptr(vis), _lib.ptr(self.tap_src), _lib.ptr(self.tap_dst),
+                  _lib.ptr(self.tap_off), _lib.ptr(img), n_steps, cfg.n_sub, cfg.per_sub, cfg.nbase,
+                  cfg.n_grid, self.n_rows, self.npad, 1.0,
+                  _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
+                  _lib.stream(self.stream))
+        return img
+
+    def push_images(self, img: torch.Tensor, n_steps: int) -> None:
+        cfg = self.cfg
+        old, new = self.rings[self.cur], self.rings[1 - self.cur]
+        _lib.call("dsx_ring_advance", _lib.ptr(old), _lib.ptr(new), _lib.ptr(img), cfg.n_sub,
+                  self.n_pix, self.hist, self.cap, self.prev_len, n_steps,
+                  _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
+                  _lib.stream(self.stream))
+        self.cur = 1 - self.cur
+        self.prev_len = n_steps
+
+    def detect(self, n_steps: int) -> BlockResult:
+        """Dedisperse + detect the block just pushed (centres [T0 - 64, T0 + L - 64))."""
+        cfg = self.cfg
+        T0 = self.T0
+        c_lo = T0 - DETECT_DELAY_STEPS
+        tile, win_total, info = self._tile_for(n_steps)
+        counts = self._block_counts(c_lo, n_steps)
+        self.blk_count.copy_(torch.from_numpy(counts), non_blocking=False)
+        self.n_cand.zero_()
+        _lib.call("dsx_dedisp_detect", _lib.ptr(self.rings[self.cur]),
+                  _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
+                  cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(self.shift), cfg.n_dm,
+                  _lib.ptr(info), _lib.ptr(self.valid_from), _lib.ptr(self.n_prev),
+                  _lib.ptr(self.blk_count), _lib.ptr(self.mean), _lib.ptr(self.m2), c_lo, n_steps,
+                  self.n_widths, tile, win_total, float(cfg.threshold), float(cfg.min_stat_count),
+                  _lib.ptr(self.cand), self.cand_cap, _lib.ptr(self.n_cand), _lib.stream(self.stream))
+        n = int(self.n_cand.item())
+        k = min(n, self.cand_cap)
+        rec = self.cand[:k].cpu().numpy().view(CAND_DTYPE).reshape(-1) if k else np.zeros(0, CAND_DTYPE)
+        cands = decode_chunk_maxima(rec, widths=cfg.widths, n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
+                                    threshold=cfg.threshold, n_top=cfg.n_top, radius_t=cfg.radius_t,
+                                    radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm, t_int=cfg.t_int)
+        self.n_prev_host += counts
+        self.n_prev.copy_(torch.from_numpy(self.n_prev_host))
+        self.T0 = T0 + n_steps
+        return BlockResult(candidates=cands, n_raw=n, overflow=n > self.cand_cap, T0=T0, n_steps=n_steps,
+                           records=rec)
+
+    def process_block(self, raw: torch.Tensor, n_samp: Optional[int] = None) -> BlockResult:
+        cfg = self.cfg
+        n_samp = cfg.n_samp if n_samp is None else int(n_samp)
+        n_steps = n_samp // cfg.t_int
+        if n_samp % (2 * cfg.t_int) or n_steps % DETECT_CHUNK_STEPS or n_steps > self.cap:
+            raise ValueError(f"block of {n_samp} samples: steps must be a multiple of "
+                             f"{DETECT_CHUNK_STEPS} and <= {self.cap}")
+        vis = self.correlate(raw, n_samp)
+        img = self.image(vis, n_steps)
+        self.push_images(img, n_steps)
+        return self.detect(n_steps)

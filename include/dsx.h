@@ -1,0 +1,192 @@
+/*
+ * dsx.h -- C ABI of libdsx.so, the MI355X (gfx950) search-node fast path:
+ * 4-bit voltage unpack -> int8-MFMA correlator -> nearest-cell gridder ->
+ * C2R FFT imager -> subband dedispersion fused with boxcar detection.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success or a negative DSX_E* code;
+ *     dsx_last_error() returns the message of the calling thread's last
+ *     failure.  No entry point allocates or frees caller memory.
+ *   - Pointers are device pointers (hipMalloc / torch CUDA tensors) unless
+ *     the name says host_.  Work is enqueued on `stream` (0 = default);
+ *     nothing synchronises unless stated.
+ *   - Visibilities are int32 (re, im) pairs: exact integer correlations of
+ *     the 4-bit samples, vis[bls] = sum conj(E_b) E_a with b <= a and
+ *     bls = a (a + 1) / 2 + b.
+ *
+ * The reference (dsa110/dsa110-rt) has no native boundary on this path: its
+ * Python stage functions call Triton / NVRTC kernels directly.  Each entry
+ * point below names the reference function whose work it replaces.
+ */
+#ifndef DSX_H
+#define DSX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *dsx_stream_t; /* a hipStream_t */
+
+enum {
+    DSX_OK = 0,
+    DSX_EINVAL = -1,   /* bad shape / argument */
+    DSX_ELAUNCH = -2,  /* kernel launch failed */
+    DSX_ERANGE = -3,   /* size beyond a documented limit */
+    DSX_ENOTSUP = -4,  /* configuration not supported by this build */
+};
+
+enum { DSX_F32 = 0, DSX_F16 = 1, DSX_I8 = 2, DSX_I32 = 3 };
+
+/* Library version, (major << 16) | (minor << 8) | patch. */
+int dsx_version(void);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *dsx_last_error(void);
+
+/* ---------------------------------------------------------------- unpack
+ * On-wire bytes (n_pkt, n_ant, n_chan, 2 t, 2 pol) -> real and imag planes
+ * in GEMM layout (n_chan, 2 t, 2 pol, n_pkt, n_ant).  out_dtype DSX_I8
+ * writes the raw 4-bit integers (scale ignored); DSX_F16 / DSX_F32 write
+ * value * scale.
+ * Replaces unpack_int4_split, src/dsart/services/slow_corr_kernel.py:220-403
+ * (Triton kernel services/triton_unpack_onwire.py:51-226). */
+int dsx_unpack_i4(const uint8_t *raw, void *re, void *im, int n_pkt, int n_ant,
+                  int n_chan, int out_dtype, float scale, dsx_stream_t stream);
+
+/* ------------------------------------------------------------ correlator
+ * Exact int8-MFMA correlation with int32 accumulation.
+ *   input_layout 0: `raw` holds on-wire bytes (n_pkt, n_ant, n_chan, 2, 2);
+ *                   `im` must be NULL.
+ *   input_layout 1: `raw`/`im` hold int8 real/imag planes in GEMM layout
+ *                   (n_chan, 2, 2, n_pkt, n_ant) (values in [-8, 7]).
+ * Each output sums t_int native samples, csf adjacent channels and, when
+ * fuse_pol != 0, both polarisations (Stokes I, no 1/2).
+ *   out_layout 0: vis (n_steps, nbase, n_chan / csf, npol_out, 2)   (reference)
+ *   out_layout 1: vis (n_steps, n_chan / csf, npol_out, nbase, 2)   (subband-major)
+ * with n_steps = 2 n_pkt / t_int and npol_out = fuse_pol ? 1 : 2.
+ * Replaces FastCorrKernel.compute_split + stokes_i_pol_sum,
+ * src/dsart/services/corr_fast_kernel.py:364-792 (Triton
+ * services/triton_corr_fused.py:44-209). */
+int dsx_corr_i8(const void *raw, const void *im, int32_t *vis, int n_pkt, int n_ant,
+                int n_chan, int t_int, int csf, int fuse_pol, int input_layout,
+                int out_layout, dsx_stream_t stream);
+
+/* --------------------------------------------------------------- gridder
+ * Gather form of the nearest-cell gridder: for every step f and filled cell
+ * c, out[f][c] = sum over sources i in [cell_ptr[c], cell_ptr[c+1]) of
+ * vis[f][cell_src[i]] where vis is (n_steps, n_src) complex.  vis_dtype
+ * DSX_I32 (int32 pairs, exact; out int32 pairs) or DSX_F32 (float pairs,
+ * e.g. complex64; out float pairs, summed in CSR order).
+ * Replaces FastVisGridder.compute, src/dsart/grid/kernel.py:508-611 (and the
+ * CSR gather of services/triton_dedisp.py:50-95). */
+int dsx_grid_gather(const void *vis, const int32_t *cell_ptr, const int32_t *cell_src,
+                    void *out, int n_steps, int n_src, int n_cells, int vis_dtype,
+                    dsx_stream_t stream);
+
+/* ---------------------------------------------------------------- imager
+ * Fused gridder + C2R FFT imager.  For every step t and subband s of the
+ * subband-major visibilities vis (n_steps, n_sub * per_sub, nbase, 2) int32,
+ * scatter taps [tap_off[s], tap_off[s+1]) (tap_off: device, n_sub + 1) into the Hermitian half plane
+ * H[r][w] (r < n_rows), inverse-FFT, and write
+ *     img[s][t][i][j] = mask01(i, j) * scale * C2R(H)[i][j] / (2 N^2)
+ * which equals edge_mask * fftshift(Re(ifft2(grid))) of the reference grid.
+ * tap_src indexes one (t, s) slab of per_sub * nbase pairs; tap_dst holds
+ * cell r * N + w with bit 20 = conjugate and bit 21 = negate.  mask01 zeroes
+ * the outer npad ring and the DC pixel (N/2, N/2).  img_dtype DSX_F16 or
+ * DSX_F32; img is (n_sub, n_steps, N, N).
+ * Replaces grid + dirty_image_from_uv_grid + apply_edge_mask,
+ * src/dsart/image/imager.py:112-324 and GpuImager.process_cube,
+ * src/dsart/image/imager_gpu.py:217-657. */
+int dsx_image_subbands(const int32_t *vis, const int32_t *tap_src, const int32_t *tap_dst,
+                       const int32_t *tap_off, void *img, int n_steps, int n_sub,
+                       int per_sub, int nbase, int n_grid, int n_rows, int npad,
+                       float scale, int img_dtype, dsx_stream_t stream);
+
+/* ------------------------------------------------------------ image ring
+ * Time-major subband image history.  ring is (n_sub, n_pix, hist + cap)
+ * with step T0 - hist + k at position k.  Copies the last `hist` steps of
+ * ring_old (which held `prev_len` new steps after its history) into
+ * ring_new[.., 0:hist), then transposes img (n_sub, n_new, n_pix) into
+ * ring_new[.., hist:hist + n_new).  dtype DSX_F16 or DSX_F32 for both.
+ * Replaces the per-chgroup stream history of combine_chgroups,
+ * src/dsart/fine_dm/combiner.py:548-655 (T_stream = T_det + max shift). */
+int dsx_ring_advance(const void *ring_old, void *ring_new, const void *img, int n_sub,
+                     int n_pix, int hist, int cap, int prev_len, int n_new, int dtype,
+                     dsx_stream_t stream);
+
+/* ------------------------------------------------ dedispersion + detector
+ * One block of centres [c_lo, c_lo + n_centres) (multiple of 64) for every
+ * DM and pixel of the time-major ring (n_sub, n_pix, ring_len), where
+ * position hist + tau - T0 holds absolute step tau:
+ *   D[d][t][p] = sum_s ring[s][p][hist + (t - shift[s][d]) - T0]
+ *   B_k(c)     = sum_{j<w_k} D[c - w_k/2 + j],   w_k = 2^k, k < n_widths
+ *   snr_k      = (B_k - w_k mean[p][d]) / (sqrt(m2[p][d] / n_prev[d]) sqrt(w_k))
+ * For each aligned chunk of 64 centres whose boxes are complete
+ * (first centre - w_max/2 >= valid_from[d]) and each DM with
+ * n_prev[d] >= min_count, the chunk maximum over widths (ties -> lowest k,
+ * then earliest centre) is appended to cand (16-byte records
+ * {int32 t, int32 p, int16 d, int16 k, float snr}) when snr > threshold,
+ * counted by *n_cand (device int32; entries past cand_cap are dropped but
+ * still counted).  Complete centres (t >= valid_from[d]) feed the block's
+ * sum / sum of squares, which are merged (Chan) into the float64 running
+ * mean / m2 ([p][d] layout) with blk_count[d] samples.  shift is [s][d]
+ * with shift[s][0] = 0.  The work-group stages per-subband windows of
+ * `tile` centres in LDS: sub_info[s] = (offset, max_d shift[s][d]) with
+ * offset_s = sum_{s' < s} (tile + 2 (w_max / 2) + max shift_s') and
+ * win_total the sum over all s; dsx_dedisp_tile() picks `tile`.
+ * ring_dtype DSX_F16 or DSX_F32; hist >= 64 + w_max/2 + max shift.
+ * Replaces combine_chgroups + boxcar_via_cumsum + multi_boxcar_argmax +
+ * the Layer-1/Layer-2 normalisation, src/dsart/fine_dm/combiner.py:548-655,
+ * src/dsart/detector/forward.py:163-504,1264-1854,
+ * src/dsart/detector/triton_boxcar.py:193-403,
+ * src/dsart/noise_norm/layer2.py:313-452. */
+int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, int n_pix, int hist,
+                      int ring_len, int64_t T0, const int32_t *shift, int n_dm,
+                      const int32_t *sub_info, const int32_t *valid_from, const double *n_prev,
+                      const double *blk_count, double *mean, double *m2, int64_t c_lo,
+                      int n_centres, int n_widths, int tile, int win_total, float threshold,
+                      double min_count, void *cand, int cand_cap, int32_t *n_cand,
+                      dsx_stream_t stream);
+
+/* Largest time tile (multiple of 64 dividing n_centres, <= 2048) whose LDS
+ * windows fit lds_budget bytes; 0 if none.  host_shmax[s] = max_d shift[s][d]. */
+int dsx_dedisp_tile(int n_sub, const int32_t *host_shmax, int n_widths, int n_centres,
+                    int lds_budget);
+
+/* ------------------------------------------------------------- synthetic
+ * Counter-based synthetic voltages for samples [sample0, sample0 + 2 n_pkt):
+ * splitmix64(seed + i * 0x9E3779B97F4A7C15) of the flat component index,
+ * top 16 bits through lut (65536 int32, 1/1024 units), plus burst phasors
+ * (n_burst, n_ant, n_chan, 2) int32 for samples in
+ * [burst_start[b][c], burst_start[b][c] + burst_len[b]), rounded half to
+ * even, clipped to [-7, 7], packed (real low nibble) in on-wire layout.
+ * Test-data generator; no reference counterpart on the path
+ * (cf. src/dsart/inject/online.py for the reference's injector). */
+int dsx_synth_block(uint8_t *raw, uint64_t seed, int64_t sample0, int n_pkt, int n_ant,
+                    int n_chan, const int32_t *lut, int n_burst, const int64_t *burst_start,
+                    const int32_t *burst_len, const int32_t *burst_phasor, dsx_stream_t stream);
+
+/* ------------------------------------------------------ detector helpers
+ * Centred boxcar along the leading axis of x (n_t, n_inner) float32:
+ * out[i] = sum_{j<w} x[i - w/2 + j] with zeros outside.
+ * Replaces boxcar_via_cumsum, src/dsart/detector/forward.py:163-247. */
+int dsx_boxcar(const float *x, float *out, int n_t, int n_inner, int width,
+               dsx_stream_t stream);
+
+/* For each output cell (n_out, n_inner) compute K boxcar scores from a
+ * padded cumulative sum cs (n_tpad, n_inner) (float32 or float16):
+ * snr_k = (cs[t_base + t + off_k + w_k] - cs[t_base + t + off_k]) * sigma_inv_k,
+ * keep the maximum (strict >, ties -> lowest k) and its k.
+ * Replaces multi_boxcar_argmax_triton, src/dsart/detector/triton_boxcar.py:193-403. */
+int dsx_multi_boxcar_argmax(const void *cs, int cs_dtype, const int32_t *widths,
+                            const int32_t *offsets, const float *sigma_inv, int n_k,
+                            void *out_max, int out_dtype, int16_t *out_win, int n_tpad,
+                            int n_out, int n_inner, int t_base, dsx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DSX_H */

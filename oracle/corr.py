@@ -1,0 +1,74 @@
+"""int64 correlator oracle (test infrastructure only).
+
+Pinned by: the exhaustive 256-byte nibble table and the pack/unpack round
+trip (``tests/test_slow_corr_synth.py:172-211`` in the reference), the sign
+known answer ``E_1 = (1, 0)``, ``E_2 = (0, 1)`` -> ``V = +iK``
+(``tests/test_slow_corr_synth.py:219-277``), zero-in/zero-out, real
+non-negative autos, and an independent complex-loop brute force on tiny
+inputs -- see ``tests/test_oracle_corr.py``.
+"""
+from __future__ import annotations
+
+from typing import Tuple
+
+import numpy as np
+
+
+def upper_tri(nants: int) -> Tuple[np.ndarray, np.ndarray]:
+    """``(a, b)`` pairs with ``b <= a`` in ``bls = a(a+1)/2 + b`` order
+    (slow_corr_kernel.py:161-178)."""
+    a_l, b_l = [], []
+    for a in range(nants):
+        for b in range(a + 1):
+            a_l.append(a)
+            b_l.append(b)
+    return np.asarray(a_l, dtype=np.int64), np.asarray(b_l, dtype=np.int64)
+
+
+def unpack_bytes(raw: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """Low nibble = real, high nibble = imag, 4-bit two's complement
+    (slow_corr_kernel.py:337-403: ``((int8)b << 4) >> 4`` and ``(int8)b >> 4``)."""
+    b = np.asarray(raw, dtype=np.uint8).astype(np.int64)
+    lo = b & 0x0F
+    hi = (b >> 4) & 0x0F
+    lo = np.where(lo >= 8, lo - 16, lo)
+    hi = np.where(hi >= 8, hi - 16, hi)
+    return lo, hi
+
+
+def voltages(raw: np.ndarray, n_pkt: int, n_ant: int, n_chan: int) -> Tuple[np.ndarray, np.ndarray]:
+    """On-wire ``(pkt, ant, chan, 2t, 2p)`` bytes -> int64 ``(sample, ant, chan, pol)``."""
+    re, im = unpack_bytes(np.asarray(raw).reshape(n_pkt, n_ant, n_chan, 2, 2))
+    re = re.transpose(0, 3, 1, 2, 4).reshape(n_pkt * 2, n_ant, n_chan, 2)
+    im = im.transpose(0, 3, 1, 2, 4).reshape(n_pkt * 2, n_ant, n_chan, 2)
+    return re, im
+
+
+def correlate(raw: np.ndarray, *, n_pkt: int, n_ant: int, n_chan: int, t_int: int,
+              csf: int = 1, fuse_pol: bool = True) -> np.ndarray:
+    """Exact integer visibilities ``V[bls] = sum conj(E_b) E_a`` (b <= a).
+
+    Sums over ``t_int`` consecutive samples, ``csf`` consecutive channels and,
+    when ``fuse_pol``, both polarisations (Stokes I = XX + YY, no 1/2,
+    corr_fast_kernel.py:764-792).  Returns int64
+    ``(n_steps, n_chan // csf, nbase, 2)`` (re, im) when ``fuse_pol`` else
+    ``(n_steps, n_chan // csf, 2, nbase, 2)``.
+    """
+    re, im = voltages(raw, n_pkt, n_ant, n_chan)
+    S = n_pkt * 2
+    n_steps = S // t_int
+    nco = n_chan // csf
+    # (step, t, cout, x, pol, ant)
+    R = re.reshape(n_steps, t_int, n_ant, nco, csf, 2).transpose(0, 1, 3, 4, 5, 2)
+    I = im.reshape(n_steps, t_int, n_ant, nco, csf, 2).transpose(0, 1, 3, 4, 5, 2)
+    if fuse_pol:
+        sub = "stcxpa,stcxpb->scab"
+    else:
+        sub = "stcxpa,stcxpb->scpab"
+    # V[a, b] = sum conj(E_b) E_a : re = Rb Ra + Ib Ia ; im = Rb Ia - Ib Ra
+    vre = np.einsum(sub, R, R) + np.einsum(sub, I, I)
+    vim = np.einsum(sub, I, R) - np.einsum(sub, R, I)
+    a_idx, b_idx = upper_tri(n_ant)
+    out_re = vre[..., a_idx, b_idx]
+    out_im = vim[..., a_idx, b_idx]
+    return np.ascontiguousarray(np.stack([out_re, out_im], axis=-1), dtype=np.int64)

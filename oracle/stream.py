@@ -1,0 +1,97 @@
+"""Block-by-block oracle of the whole fast path (test infrastructure only).
+
+Each block: int64 correlation (``oracle.corr``), np.add.at gridding +
+``fftshift(Re(ifft2))`` imaging (``oracle.image``), a float64 image history,
+dedispersion + detection against the running per-(DM, pixel) statistics from
+previous blocks (``oracle.search``), then the Chan merge of this block's
+statistics.  A DM is scored only once it holds ``min_stat_count`` complete
+samples of statistics from earlier blocks; before that its blocks only prime
+the statistics, like the reference Layer-2 burn-in
+(``src/dsart/noise_norm/layer2.py:313-452``).
+
+``cfg`` is duck-typed: ``n_ant, chan_freqs_GHz, t_int, csf, n_sub, n_grid,
+grid_freq_GHz, cell_lambda, dm_max, n_dm, widths, threshold, n_top,
+radius_t, radius_dm, radius_lm``.
+"""
+from __future__ import annotations
+
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import corr as ocorr
+from . import image as oimage
+from . import search as osearch
+
+DELAY = 64
+CHUNK = 64
+
+
+class OracleStream:
+    def __init__(self, cfg, antpos_e, antpos_n, *, pix: Optional[np.ndarray] = None):
+        self.cfg = cfg
+        f = np.asarray(cfg.chan_freqs_GHz, dtype=np.float64)
+        nco = f.size // cfg.csf
+        self.freq_out = f.reshape(nco, cfg.csf).mean(axis=1) if cfg.csf > 1 else f
+        gf = self.freq_out if cfg.grid_freq_GHz is None else np.full(nco, float(cfg.grid_freq_GHz))
+        self.rows, self.cols, self.cell_lambda = oimage.cell_map(
+            antpos_e, antpos_n, gf, cfg.n_grid, cfg.cell_lambda)
+        per = nco // cfg.n_sub
+        nu_ref = self.freq_out[::per][: cfg.n_sub]
+        t_step = cfg.t_int * osearch.NATIVE_US
+        self.dm = np.linspace(0.0, float(cfg.dm_max), int(cfg.n_dm))
+        self.shifts = osearch.shift_table(self.dm, nu_ref, float(f[-1]), t_step)
+        self.mask = oimage.edge_mask(cfg.n_grid, 1)
+        n_pix_full = cfg.n_grid * cfg.n_grid
+        self.pix = np.arange(n_pix_full) if pix is None else np.asarray(pix, dtype=np.int64)
+        npx = self.pix.size
+        self.widths = list(cfg.widths)
+        self.half = max(self.widths) // 2
+        self.st_n = np.zeros(cfg.n_dm)
+        self.st_mean = np.zeros((cfg.n_dm, npx))
+        self.st_m2 = np.zeros((cfg.n_dm, npx))
+        self.hist = np.zeros((cfg.n_sub, 0, npx))
+        self.hist_tau0 = 0
+        self.t_next = 0                  # first step of the next block
+        self.keep = int(self.shifts.max()) + self.half + DELAY + 1
+
+    # -- stages ---------------------------------------------------------------
+    def visibilities(self, raw: np.ndarray, n_samp: int) -> np.ndarray:
+        c = self.cfg
+        return ocorr.correlate(raw, n_pkt=n_samp // 2, n_ant=c.n_ant, n_chan=len(c.chan_freqs_GHz),
+                               t_int=c.t_int, csf=c.csf, fuse_pol=True)
+
+    def images(self, vis: np.ndarray) -> np.ndarray:
+        """``(n_sub, n_steps, n_pix_sel)`` float64."""
+        c = self.cfg
+        img = oimage.subband_images(vis, self.rows, self.cols, c.n_grid, c.n_sub, self.mask)
+        return img.reshape(c.n_sub, img.shape[1], -1)[:, :, self.pix]
+
+    def push_images(self, img: np.ndarray):
+        """Append a block of images; run detection; merge statistics."""
+        c = self.cfg
+        L = img.shape[1]
+        T0 = self.t_next
+        self.hist = np.concatenate([self.hist, img], axis=1)
+        drop = max(0, self.hist.shape[1] - (self.keep + L))
+        if drop:
+            self.hist = self.hist[:, drop:]
+            self.hist_tau0 += drop
+        self.t_next = T0 + L
+        c_lo = T0 - DELAY
+        bs, bt, bk, s1, s2, cnt = osearch.search_block_c(
+            self.hist, self.hist_tau0, self.shifts, c_lo, L, CHUNK, self.widths,
+            self.st_n, self.st_mean, self.st_m2, getattr(c, "min_stat_count", 512))
+        self.last = dict(best_snr=bs, best_t=bt, best_k=bk, blk_sum=s1, blk_sumsq=s2, count=cnt,
+                         c_lo=c_lo)
+        cands = osearch.decode(bs, bt, bk, self.widths, c.n_grid if self.pix.size == c.n_grid ** 2 else 1,
+                               threshold=c.threshold, n_top=c.n_top, radius_t=c.radius_t,
+                               radius_dm=c.radius_dm, radius_lm=c.radius_lm) \
+            if self.pix.size == c.n_grid ** 2 else []
+        self.st_n, self.st_mean, self.st_m2 = osearch.merge_stats(
+            self.st_n, self.st_mean, self.st_m2, cnt, s1, s2)
+        return cands
+
+    def push_raw(self, raw: np.ndarray, n_samp: int):
+        vis = self.visibilities(raw, n_samp)
+        return self.push_images(self.images(vis))

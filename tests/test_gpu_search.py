@@ -1,0 +1,101 @@
+"""GPU dedispersion + detection parity against the float64 oracle, and the
+whole config-1 stream (16 ant x 64 ch, 7 blocks, DM-250 burst) end to end.
+
+Bars: chunk-maximum records identical in (DM, chunk, pixel, time, width)
+with S/N within 1e-4 relative for float32 images; decoded candidate keys
+(DM index, time bin, width, row, col) identical with S/N within 2e-3
+relative; float64 statistics within 1e-5 relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+from dsx import config as dcfg
+from dsx import synth
+from dsx.pipeline import SearchNode
+from oracle.stream import OracleStream
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_records(o, thr):
+    L = o.last
+    d, c, p = np.nonzero(L["best_snr"] > thr)
+    return {(int(dd), int(L["best_t"][dd, cc, pp]), int(pp)): (int(L["best_k"][dd, cc, pp]),
+                                                                  float(L["best_snr"][dd, cc, pp]))
+            for dd, cc, pp in zip(d, c, p)}
+
+
+def _gpu_records(rec):
+    return {(int(r["d"]), int(r["t"]), int(r["p"])): (int(r["k"]), float(r["snr"])) for r in rec}
+
+
+def _compare_records(g, o, thr, rtol):
+    near = lambda v: abs(v - thr) < 1e-3 * max(1.0, abs(thr))
+    gk = {k for k, v in g.items() if not near(v[1])}
+    ok = {k for k, v in o.items() if not near(v[1])}
+    assert gk == ok, f"record sets differ: gpu-only {sorted(gk - ok)[:5]} oracle-only {sorted(ok - gk)[:5]}"
+    for k in gk:
+        assert g[k][0] == o[k][0], f"{k}: width {g[k][0]} vs {o[k][0]}"
+        assert abs(g[k][1] - o[k][1]) <= rtol * abs(o[k][1]), f"{k}: snr {g[k][1]} vs {o[k][1]}"
+
+
+def _cand_keys(cands):
+    return [(c.dm_idx, c.time_bin, c.width_samples, int(c.l), int(c.m)) for c in cands]
+
+
+def test_dedisp_detect_random_images_vs_oracle():
+    cfg = dcfg.c1_config().with_(n_sub=8, n_grid=16, dm_max=100.0, n_dm=64, threshold=4.5,
+                                 min_stat_count=256, widths=(1, 2, 4, 8, 16))
+    rng = np.random.default_rng(123)
+    ant_e = rng.uniform(-500, 500, cfg.n_ant)
+    ant_n = rng.uniform(-100, 100, cfg.n_ant)
+    node = SearchNode(cfg, ant_e, ant_n, ring_dtype=torch.float32)
+    o = OracleStream(cfg, ant_e, ant_n)
+    sh = cfg.shift_table()
+    L = cfg.n_steps
+    N = cfg.n_grid
+    total = 0
+    for blk in range(5):
+        img = rng.standard_normal((cfg.n_sub, L, N * N)).astype(np.float32)
+        if blk >= 3:       # dispersed pulses at DM index 40, pixel 77, width 4
+            t0 = blk * L + 100
+            for s in range(cfg.n_sub):
+                tt = t0 - blk * L - int(sh[40, s])
+                if 0 <= tt < L - 4:
+                    img[s, tt:tt + 4, 77] += 2.5
+        img_t = torch.from_numpy(img.reshape(cfg.n_sub, L, N, N)).cuda()
+        node.push_images(img_t, L)
+        res = node.detect(L)
+        o.push_images(img.astype(np.float64))
+        g = _gpu_records(res.records)
+        ro = _oracle_records(o, cfg.threshold)
+        _compare_records(g, ro, cfg.threshold, 1e-4)
+        total += len(g)
+        mean = node.mean.cpu().numpy().T
+        m2 = node.m2.cpu().numpy().T
+        np.testing.assert_allclose(mean, o.st_mean, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(m2, o.st_m2, rtol=1e-5, atol=1e-5)
+    assert total > 0
+
+
+@pytest.mark.parametrize("ring_dtype", [torch.float16, torch.float32])
+def test_c1_stream_candidates_match_oracle(ring_dtype):
+    cfg = dcfg.c1_config()
+    sc = synth.make_scenario(cfg, synth.c1_bursts(cfg))
+    gs = synth.GpuSynth(sc)
+    node = SearchNode(cfg, sc.antpos_e, sc.antpos_n, ring_dtype=ring_dtype)
+    o = OracleStream(cfg, sc.antpos_e, sc.antpos_n)
+    found = []
+    for blk in range(7):
+        print(f"block {blk}", flush=True)
+        s0 = blk * cfg.n_samp
+        res = node.process_block(gs.block(s0, cfg.n_samp), cfg.n_samp)
+        want = o.push_raw(sc.block(s0, cfg.n_samp), cfg.n_samp)
+        gk, ok = _cand_keys(res.candidates), [(c.dm_idx, c.t, c.width, c.row, c.col) for c in want]
+        assert gk == ok, f"block {blk}: gpu {gk} oracle {ok}"
+        for gc, oc in zip(res.candidates, want):
+            assert abs(gc.snr - oc.snr) <= 2e-3 * abs(oc.snr), (gc, oc)
+        found += res.candidates
+    b = sc.bursts[0]
+    assert any(abs(c.l - b.row) <= 1 and abs(c.m - b.col) <= 1 and c.snr > 15 for c in found), found
