@@ -101,6 +101,23 @@ class SearchNode:
         self.tiles: Dict[int, tuple] = {}
         self.T0 = 0
         self.prev_len = 0
+        self.marks: Optional[Dict[str, list]] = None     # stage -> [start, end] events when timing
+
+    def enable_timing(self) -> None:
+        """Record HIP events around every kernel (read with :meth:`stage_ms`)."""
+        names = ("corr", "image", "ring", "dedisp")
+        self.marks = {n: [torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)]
+                      for n in names}
+
+    def _mark(self, name: str, which: int) -> None:
+        if self.marks is not None:
+            s = self.stream if self.stream is not None else torch.cuda.current_stream()
+            self.marks[name][which].record(s)
+
+    def stage_ms(self) -> Dict[str, float]:
+        """Kernel times of the last block (synchronises)."""
+        torch.cuda.synchronize()
+        return {n: e[0].elapsed_time(e[1]) for n, e in (self.marks or {}).items()}
 
     # ------------------------------------------------------------------ helpers
     def _tile_for(self, n_centres: int):
@@ -134,28 +151,34 @@ class SearchNode:
         if raw.dtype != torch.uint8 or raw.numel() != n_pkt * cfg.n_ant * cfg.n_chan * 4:
             raise ValueError("raw must be uint8 of shape (n_pkt, n_ant, n_chan, 2, 2)")
         vis = self.vis[:n_steps]
+        self._mark("corr", 0)
         _lib.call("dsx_corr_i8", _lib.ptr(raw), None, _lib.ptr(vis), n_pkt, cfg.n_ant, cfg.n_chan,
                   cfg.t_int, cfg.csf, 1, 0, 1, _lib.stream(self.stream))
+        self._mark("corr", 1)
         return vis
 
     def image(self, vis: torch.Tensor, n_steps: int) -> torch.Tensor:
         cfg = self.cfg
         img = self.img[:, :n_steps] if n_steps == self.cap else \
             torch.empty((cfg.n_sub, n_steps, cfg.n_grid, cfg.n_grid), dtype=self.ring_dtype, device=self.dev)
+        self._mark("image", 0)
         _lib.call("dsx_image_subbands", _lib.ptr(vis), _lib.ptr(self.tap_src), _lib.ptr(self.tap_dst),
                   _lib.ptr(self.tap_off), _lib.ptr(img), n_steps, cfg.n_sub, cfg.per_sub, cfg.nbase,
                   cfg.n_grid, self.n_rows, self.npad, 1.0,
                   _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
                   _lib.stream(self.stream))
+        self._mark("image", 1)
         return img
 
     def push_images(self, img: torch.Tensor, n_steps: int) -> None:
         cfg = self.cfg
         old, new = self.rings[self.cur], self.rings[1 - self.cur]
+        self._mark("ring", 0)
         _lib.call("dsx_ring_advance", _lib.ptr(old), _lib.ptr(new), _lib.ptr(img), cfg.n_sub,
                   self.n_pix, self.hist, self.cap, self.prev_len, n_steps,
                   _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
                   _lib.stream(self.stream))
+        self._mark("ring", 1)
         self.cur = 1 - self.cur
         self.prev_len = n_steps
 
@@ -168,6 +191,7 @@ class SearchNode:
         counts = self._block_counts(c_lo, n_steps)
         self.blk_count.copy_(torch.from_numpy(counts), non_blocking=False)
         self.n_cand.zero_()
+        self._mark("dedisp", 0)
         _lib.call("dsx_dedisp_detect", _lib.ptr(self.rings[self.cur]),
                   _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
                   cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(self.shift), cfg.n_dm,
@@ -175,6 +199,7 @@ class SearchNode:
                   _lib.ptr(self.blk_count), _lib.ptr(self.mean), _lib.ptr(self.m2), c_lo, n_steps,
                   self.n_widths, tile, win_total, float(cfg.threshold), float(cfg.min_stat_count),
                   _lib.ptr(self.cand), self.cand_cap, _lib.ptr(self.n_cand), _lib.stream(self.stream))
+        self._mark("dedisp", 1)
         n = int(self.n_cand.item())
         k = min(n, self.cand_cap)
         rec = self.cand[:k].cpu().numpy().view(CAND_DTYPE).reshape(-1) if k else np.zeros(0, CAND_DTYPE)
