@@ -28,9 +28,11 @@ struct ImgParams {
     const int32_t *tap_src;
     const int32_t *tap_dst;
     const int32_t *tap_off;   // n_sub + 1 CSR offsets (device)
+    const float2 *grid;       // dense mode: (n_items, N, N) complex64
     void *img;
     int n_steps, n_sub, per_sub, nbase, N, log2N, n_rows, rc_max, npad;
     float scale;
+    int dense;                // 0: taps from int32 visibilities (+ mask); 1: dense grid, no mask
 };
 
 template <typename TOut>
@@ -59,13 +61,29 @@ __global__ __launch_bounds__(IMG_THREADS) void image_kernel(ImgParams P) {
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
         const int s = item / P.n_steps;
         const int t = item - s * P.n_steps;
-        const int32_t *vslab = P.vis + 2 * ((int64_t)t * n_cout + (int64_t)s * P.per_sub) * P.nbase;
-        const int tap_lo = P.tap_off[s], tap_hi = P.tap_off[s + 1];
+        const int32_t *vslab = P.dense ? nullptr
+                                       : P.vis + 2 * ((int64_t)t * n_cout + (int64_t)s * P.per_sub) * P.nbase;
+        const int tap_lo = P.dense ? 0 : P.tap_off[s], tap_hi = P.dense ? 0 : P.tap_off[s + 1];
+        const float2 *gsrc = P.dense ? P.grid + (int64_t)item * N * N : nullptr;
         TOut *out = img + ((int64_t)s * P.n_steps + t) * N * N;
         for (int ck = 0; ck < n_chunks; ++ck) {
             const int r0 = ck * P.rc_max;
             const int rc = min(P.rc_max, P.n_rows - r0);
             __syncthreads();
+            if (P.dense) {
+                // Hermitian fold of (-1)^(a+b) G[a][b]: H[r][w] = (-1)^(r+w) (G[r][w] + conj G[-r][-w]),
+                // written straight into float32 rows in bit-reversed order
+                for (int i = tid; i < rc * N; i += IMG_THREADS) {
+                    const int r = i >> P.log2N, w = i & (N - 1);
+                    const int rg = r0 + r;
+                    const float2 a = gsrc[(int64_t)rg * N + w];
+                    const float2 b = gsrc[(int64_t)((N - rg) & (N - 1)) * N + ((N - w) & (N - 1))];
+                    const float sg = ((rg + w) & 1) ? -1.f : 1.f;
+                    const int wr = (int)(__brev((unsigned)w) >> (32 - P.log2N));
+                    H[r * N + wr] = make_float2(sg * (a.x + b.x), sg * (a.y - b.y));
+                }
+                __syncthreads();
+            } else {
             for (int i = tid; i < rc * N; i += IMG_THREADS) Hi[i] = make_int2(0, 0);
             __syncthreads();
             // ---- 1. scatter taps (int32 LDS atomics)
@@ -97,6 +115,7 @@ __global__ __launch_bounds__(IMG_THREADS) void image_kernel(ImgParams P) {
                 }
             }
             __syncthreads();
+            }
             // ---- 2. radix-2 DIT row transforms
             const int halfN = N >> 1;
             for (int m = 1; m < N; m <<= 1) {
@@ -158,8 +177,8 @@ __global__ __launch_bounds__(IMG_THREADS) void image_kernel(ImgParams P) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int i = i0 + 4 * lg + v;
-                        const bool inside = i >= P.npad && i < N - P.npad && j >= P.npad &&
-                                            j < N - P.npad && !(i == halfN && j == halfN);
+                        const bool inside = P.dense || (i >= P.npad && i < N - P.npad && j >= P.npad &&
+                                                        j < N - P.npad && !(i == halfN && j == halfN));
                         float val = inside ? acc[v] * out_scale : 0.f;
                         const int64_t o = (int64_t)i * N + j;
                         if (ck > 0) val += ld_as_float(out, o);
@@ -221,8 +240,8 @@ extern "C" int dsx_image_subbands(const int32_t *vis, const int32_t *tap_src,
                          "dsx_image_subbands: %d uv rows need %d passes; multi-pass imaging "
                          "accumulates in place and needs img_dtype DSX_F32", n_rows, n_chunks);
     hipStream_t s = as_stream(stream);
-    ImgParams P{vis, tap_src, tap_dst, tap_off, img, n_steps, n_sub, per_sub, nbase, n_grid,
-                log2N, n_rows, rc_max, npad, scale};
+    ImgParams P{vis, tap_src, tap_dst, tap_off, nullptr, img, n_steps, n_sub, per_sub, nbase, n_grid,
+                log2N, n_rows, rc_max, npad, scale, 0};
     const size_t lds = (size_t)n_grid * sizeof(float2) * (1 + rc_max);
     const int grid = std::min(n_sub * n_steps, 4096);
     if (img_dtype == DSX_F16) {
@@ -255,4 +274,26 @@ extern "C" int dsx_grid_gather(const void *vis, const int32_t *cell_ptr, const i
     else
         return set_error(DSX_EINVAL, "dsx_grid_gather: vis_dtype must be DSX_I32 or DSX_F32");
     return check_launch("dsx_grid_gather");
+}
+
+extern "C" int dsx_dirty_image(const void *grid, void *img, int n_images, int n_grid, float prescale,
+                               dsx_stream_t stream) {
+    DSX_REQUIRE(grid && img, "dsx_dirty_image: null pointer");
+    DSX_REQUIRE(n_images > 0, "dsx_dirty_image: n_images must be > 0");
+    DSX_REQUIRE(n_grid >= 16 && n_grid <= 1024 && (n_grid & (n_grid - 1)) == 0,
+                "dsx_dirty_image: n_grid must be a power of two in [16, 1024]");
+    int log2N = 0;
+    while ((1 << log2N) < n_grid) ++log2N;
+    const int n_rows = n_grid / 2 + 1;
+    const size_t budget = 96 * 1024;
+    int rc_max = (int)((budget - (size_t)n_grid * sizeof(float2)) / ((size_t)n_grid * sizeof(float2)));
+    rc_max = std::min(std::min(rc_max, 2 * IMG_MAX_KB), n_rows);
+    if (rc_max < 1) return set_error(DSX_ERANGE, "dsx_dirty_image: n_grid too large");
+    ImgParams P{nullptr, nullptr, nullptr, nullptr, (const float2 *)grid, img, n_images, 1, 1, 1,
+                n_grid, log2N, n_rows, rc_max, 0, prescale, 1};
+    const size_t lds = (size_t)n_grid * sizeof(float2) * (1 + rc_max);
+    (void)hipFuncSetAttribute((const void *)image_kernel<float>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    image_kernel<float><<<std::min(n_images, 4096), IMG_THREADS, lds, as_stream(stream)>>>(P);
+    return check_launch("dsx_dirty_image");
 }

@@ -103,3 +103,22 @@ class SubbandImager:
                   N, self.n_rows, self.npad, self.scale,
                   _lib.DSX_F16 if out.dtype == torch.float16 else _lib.DSX_F32, _lib.stream())
         return out
+
+
+def dirty_image_from_uv_grid(uv_grid, *, out_dtype=None, prescale: float = 1.0):
+    """``fftshift(Re(ifft2(prescale * uv_grid)))`` on the last two axes
+    (imager.py:248-301).  CUDA tensors run the HIP imager (``dsx_dirty_image``);
+    leading axes pass through.  ``out_dtype`` defaults to float32."""
+    if not isinstance(uv_grid, torch.Tensor):
+        raise TypeError("dirty_image_from_uv_grid runs on the GPU; pass a CUDA tensor")
+    if uv_grid.dim() < 2 or uv_grid.shape[-1] != uv_grid.shape[-2]:
+        raise ValueError(f"uv_grid must be [..., N, N]; got {tuple(uv_grid.shape)}")
+    N = int(uv_grid.shape[-1])
+    lead = tuple(uv_grid.shape[:-2])
+    g = uv_grid.to(torch.complex64).reshape(-1, N, N).contiguous()
+    out = torch.empty(g.shape, dtype=torch.float32, device=g.device)
+    _lib.require_gpu(g, out)
+    _lib.call("dsx_dirty_image", _lib.ptr(torch.view_as_real(g)), _lib.ptr(out), int(g.shape[0]), N,
+              float(prescale), _lib.stream())
+    out = out.reshape(lead + (N, N))
+    return out if out_dtype is None else out.to(out_dtype)

@@ -104,6 +104,13 @@ int dsx_image_subbands(const int32_t *vis, const int32_t *tap_src, const int32_t
                        int per_sub, int nbase, int n_grid, int n_rows, int npad,
                        float scale, int img_dtype, dsx_stream_t stream);
 
+/* Dense-grid imager: img[b] = fftshift(Re(ifft2(prescale * grid[b]))) for
+ * n_images complex64 grids (n_images, N, N) -> float32 (n_images, N, N),
+ * numpy's 1/N^2 normalisation, no mask.
+ * Replaces dirty_image_from_uv_grid, src/dsart/image/imager.py:248-301. */
+int dsx_dirty_image(const void *grid, void *img, int n_images, int n_grid, float prescale,
+                    dsx_stream_t stream);
+
 /* ------------------------------------------------------------ image ring
  * Time-major subband image history.  ring is (n_sub, n_pix, hist + cap)
  * with step T0 - hist + k at position k.  Copies the last `hist` steps of
@@ -154,6 +161,14 @@ int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, int n_pix, in
  * windows fit lds_budget bytes; 0 if none.  host_shmax[s] = max_d shift[s][d]. */
 int dsx_dedisp_tile(int n_sub, const int32_t *host_shmax, int n_widths, int n_centres,
                     int lds_budget);
+
+/* Shift-and-add combine of n_streams float32 streams (n_streams, t_stream,
+ * n_inner): out[t][k] = sum_g streams[g][t0 + t - shift[g]][k] for
+ * t < n_t, zero where t0 + t - shift[g] is outside [0, t_stream); shift is
+ * a device int32 array.  Complex streams pass as 2 floats per cell.
+ * Replaces combine_chgroups, src/dsart/fine_dm/combiner.py:548-655. */
+int dsx_combine(const float *streams, const int32_t *shift, float *out, int n_streams,
+                int t_stream, int64_t n_inner, int t0, int n_t, dsx_stream_t stream);
 
 /* ------------------------------------------------------------- synthetic
  * Counter-based synthetic voltages for samples [sample0, sample0 + 2 n_pkt):

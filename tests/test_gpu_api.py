@@ -1,0 +1,164 @@
+"""GPU tests of the reference-named stage APIs against the oracle / numpy.
+
+Bars: integer paths bit-exact; float32 paths within 1e-5 of the float64
+result relative to its peak (1e-3 where the inputs are float16)."""
+import numpy as np
+import pytest
+import torch
+
+from dsx import config as dcfg
+from dsx import synth
+from dsx.corr import FastCorrKernel, stokes_i_pol_sum, unpack_int4_split
+from dsx.dedisp import combine_chgroups, compute_time_shift_search
+from dsx.detector import boxcar_via_cumsum, multi_boxcar_argmax
+from dsx.grid import FastVisGridder, build_pattern
+from dsx.imager import dirty_image_from_uv_grid
+from oracle import corr as ocorr
+from oracle import image as oimage
+from oracle import search as osearch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1():
+    cfg = dcfg.c1_config().with_(n_samp=256)
+    sc = synth.make_scenario(cfg)
+    return cfg, sc, sc.block(0, 256)
+
+
+@pytest.mark.parametrize("dtype", [torch.int8, torch.float16, torch.float32])
+def test_unpack_int4_split(c1, dtype):
+    cfg, sc, raw = c1
+    re, im = unpack_int4_split(raw, device="cuda", scale=0.05, out_dtype=dtype, shape=(128, 16, 64))
+    assert re.shape == (64, 2, 2, 128, 16)
+    lo, hi = ocorr.unpack_bytes(raw)                      # (pkt, ant, ch, t, p)
+    want_re = np.transpose(lo, (2, 3, 4, 0, 1))
+    want_im = np.transpose(hi, (2, 3, 4, 0, 1))
+    if dtype == torch.int8:
+        assert np.array_equal(re.cpu().numpy(), want_re) and np.array_equal(im.cpu().numpy(), want_im)
+    else:
+        np.testing.assert_allclose(re.float().cpu().numpy(), want_re * np.float32(0.05), rtol=1e-3)
+        np.testing.assert_allclose(im.float().cpu().numpy(), want_im * np.float32(0.05), rtol=1e-3)
+
+
+def test_unpack_rejects_wrong_size(c1):
+    cfg, sc, raw = c1
+    with pytest.raises(ValueError):
+        unpack_int4_split(raw[:10], shape=(128, 16, 64))
+
+
+@pytest.mark.parametrize("fuse,csf", [(False, 1), (True, 1), (True, 4)])
+def test_fast_corr_compute_split_exact(c1, fuse, csf):
+    cfg, sc, raw = c1
+    re, im = unpack_int4_split(raw, out_dtype=torch.int8, shape=(128, 16, 64))
+    k = FastCorrKernel(device="cuda", t_int_fast_native=8, nants=16, nchan=64, accum_dtype=torch.int32)
+    got = k.compute_split(re, im, fuse_stokes_i=fuse, chan_sum_factor=csf).cpu().numpy()
+    want = ocorr.correlate(raw, n_pkt=128, n_ant=16, n_chan=64, t_int=8, csf=csf, fuse_pol=fuse)
+    want = np.transpose(want, (0, 2, 1, 3) if fuse else (0, 3, 1, 2, 4))   # -> (fv, nbase, nch, [pol,] 2)
+    assert np.array_equal(got.astype(np.int64), want)
+    raw_t = torch.from_numpy(raw).cuda()
+    got_raw = k.compute_raw(raw_t, fuse_stokes_i=fuse, chan_sum_factor=csf).cpu().numpy()
+    assert np.array_equal(got_raw, got)
+
+
+def test_fast_corr_float_path_and_stokes(c1):
+    cfg, sc, raw = c1
+    re, im = unpack_int4_split(raw, out_dtype=torch.float32, scale=0.05, shape=(128, 16, 64))
+    k = FastCorrKernel(device="cuda", t_int_fast_native=8, nants=16, nchan=64)
+    v2 = k.compute_split(re, im)                            # (fv, nbase, nch, 2) complex64
+    vi = k.compute_split(re, im, fuse_stokes_i=True)
+    want = ocorr.correlate(raw, n_pkt=128, n_ant=16, n_chan=64, t_int=8, fuse_pol=False)
+    w = (want[..., 0] + 1j * want[..., 1]).transpose(0, 3, 1, 2) * np.float32(0.05) ** 2
+    np.testing.assert_allclose(v2.cpu().numpy(), w, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(stokes_i_pol_sum(v2).cpu().numpy(), vi.cpu().numpy(), rtol=1e-6, atol=1e-6)
+
+
+def test_pattern_and_gridder(c1):
+    cfg, sc, raw = c1
+    f = np.asarray(cfg.chan_freqs_GHz)
+    pat = build_pattern(sc.antpos_e, sc.antpos_n, chgroup=0, dec_deg=71.6, n_grid=64, chan_freqs_GHz=f)
+    g = FastVisGridder.from_pattern(pat, sc.antpos_e, sc.antpos_n, device="cuda")
+    vis = ocorr.correlate(raw, n_pkt=128, n_ant=16, n_chan=64, t_int=8)            # (fv, ch, nbase, 2)
+    v_i32 = torch.from_numpy(np.ascontiguousarray(vis.transpose(0, 2, 1, 3)).astype(np.int32)).cuda()
+    got = g.compute(v_i32).cpu().numpy()
+    rows, cols, cell = oimage.cell_map(sc.antpos_e, sc.antpos_n, f, 64)
+    assert cell == pat.cell_lambda
+    vc = vis[..., 0] + 1j * vis[..., 1]
+    G = oimage.grid_plane(np.transpose(vc, (0, 2, 1)), rows, cols, 64).reshape(vis.shape[0], -1)
+    lin = pat.ix_row.astype(np.int64) * 64 + pat.ix_col
+    assert np.array_equal(got[..., 0], G[:, lin].real.astype(np.int64))
+    assert np.array_equal(got[..., 1], G[:, lin].imag.astype(np.int64))
+    assert np.abs(G).sum() == np.abs(G[:, lin]).sum()          # no value outside the pattern
+    gc = g.compute(torch.complex(v_i32[..., 0].float(), v_i32[..., 1].float())).cpu().numpy()
+    np.testing.assert_allclose(gc, G[:, lin], rtol=1e-6, atol=1e-3)
+    w = g.cell_weights.cpu().numpy()
+    assert w.sum() == (rows >= 0).sum()
+
+
+@pytest.mark.parametrize("N,B", [(64, 5), (256, 3)])
+def test_dirty_image_from_uv_grid(N, B):
+    rng = np.random.default_rng(N)
+    G = (rng.standard_normal((B, N, N)) + 1j * rng.standard_normal((B, N, N))).astype(np.complex64)
+    got = dirty_image_from_uv_grid(torch.from_numpy(G).cuda(), prescale=0.5).cpu().numpy()
+    want = np.fft.fftshift(np.fft.ifft2(0.5 * G.astype(np.complex128), axes=(-2, -1)).real, axes=(-2, -1))
+    assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+
+
+def test_combine_chgroups_vs_numpy():
+    rng = np.random.default_rng(4)
+    T, N, G = 40, 16, 5
+    streams = {g: (rng.standard_normal((T, N, N)) + 1j * rng.standard_normal((T, N, N))).astype(np.complex64)
+               for g in (0, 1, 3, 4)}                           # stream 2 missing
+    shifts = np.array([7, 5, 3, 2, 0], dtype=np.int32)
+    got = combine_chgroups(per_chgroup_streams={g: torch.from_numpy(s).cuda() for g, s in streams.items()},
+                           time_shift_per_chgroup=shifts, t_window=(3, 38)).cpu().numpy()
+    want = np.zeros((35, N, N), dtype=np.complex128)
+    for g, s in streams.items():
+        for i, t in enumerate(range(3, 38)):
+            if 0 <= t - shifts[g] < T:
+                want[i] += s[t - shifts[g]]
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5)
+
+
+def test_time_shift_table_matches_oracle():
+    cfg = dcfg.c2_config()
+    tab = compute_time_shift_search(dm_pc_cm3=cfg.dm_trials, nu_subband_ref_GHz=cfg.subband_ref_GHz,
+                                    nu_bot_GHz=cfg.nu_bot_GHz, t_int_search_us=cfg.t_step_us)
+    assert np.array_equal(tab.shifts, osearch.shift_table(cfg.dm_trials, cfg.subband_ref_GHz,
+                                                          cfg.nu_bot_GHz, cfg.t_step_us))
+
+
+@pytest.mark.parametrize("w", [1, 2, 5, 8, 32])
+def test_boxcar_via_cumsum(w):
+    rng = np.random.default_rng(20260505)
+    x = rng.standard_normal((64, 3, 4, 4)).astype(np.float32)
+    got = boxcar_via_cumsum(torch.from_numpy(x).cuda(), axis=0, width=w)
+    got = got.cpu().numpy()
+    np.testing.assert_allclose(got, osearch.boxcar_np(x.astype(np.float64), w), rtol=1e-5, atol=1e-5)
+    y = rng.standard_normal((4, 32, 5)).astype(np.float32)
+    got1 = boxcar_via_cumsum(torch.from_numpy(y).cuda(), axis=1, width=min(w, 32)).cpu().numpy()
+    np.testing.assert_allclose(got1, osearch.boxcar_np(y.astype(np.float64), min(w, 32), axis=1),
+                               rtol=1e-5, atol=1e-5)
+
+
+def test_multi_boxcar_argmax():
+    rng = np.random.default_rng(7)
+    T, F, H, W, maxw = 96, 2, 4, 4, 16
+    x = rng.standard_normal((T, F, H, W))
+    pl = maxw // 2
+    cs = np.zeros((T + maxw, F, H, W))
+    cs[1 + pl:1 + pl + T] = np.cumsum(x, axis=0)
+    cs[1 + pl + T:] = cs[pl + T]
+    widths = np.array([1, 2, 4, 8, 16], dtype=np.int32)
+    offs = (pl - widths // 2).astype(np.int32)
+    sinv = (1.0 / np.sqrt(widths)).astype(np.float32)
+    cs32 = cs.astype(np.float32)
+    mx, win = multi_boxcar_argmax(torch.from_numpy(cs32).cuda(), widths=torch.from_numpy(widths).cuda(),
+                                  offsets=torch.from_numpy(offs).cuda(), sigma_inv=torch.from_numpy(sinv).cuda(),
+                                  n_out=T - maxw)
+    scores = np.stack([(cs32[np.arange(T - maxw)[:, None] + o + w].reshape(T - maxw, -1)
+                        - cs32[np.arange(T - maxw)[:, None] + o].reshape(T - maxw, -1)) * s
+                       for w, o, s in zip(widths, offs, sinv)])             # (K, n_out, inner)
+    np.testing.assert_allclose(mx.cpu().numpy().reshape(T - maxw, -1), scores.max(axis=0), rtol=1e-6, atol=1e-6)
+    assert np.array_equal(win.cpu().numpy().reshape(T - maxw, -1), scores.argmax(axis=0))
