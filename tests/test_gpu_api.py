@@ -9,7 +9,7 @@ import torch
 from dsx import config as dcfg
 from dsx import synth
 from dsx.corr import FastCorrKernel, stokes_i_pol_sum, unpack_int4_split
-from dsx.dedisp import combine_chgroups, compute_time_shift_search
+from dsx.dedisp import combine_chgroups
 from dsx.detector import boxcar_via_cumsum, multi_boxcar_argmax
 from dsx.grid import FastVisGridder, build_pattern
 from dsx.imager import dirty_image_from_uv_grid
@@ -119,14 +119,6 @@ def test_combine_chgroups_vs_numpy():
             if 0 <= t - shifts[g] < T:
                 want[i] += s[t - shifts[g]]
     np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5)
-
-
-def test_time_shift_table_matches_oracle():
-    cfg = dcfg.c2_config()
-    tab = compute_time_shift_search(dm_pc_cm3=cfg.dm_trials, nu_subband_ref_GHz=cfg.subband_ref_GHz,
-                                    nu_bot_GHz=cfg.nu_bot_GHz, t_int_search_us=cfg.t_step_us)
-    assert np.array_equal(tab.shifts, osearch.shift_table(cfg.dm_trials, cfg.subband_ref_GHz,
-                                                          cfg.nu_bot_GHz, cfg.t_step_us))
 
 
 @pytest.mark.parametrize("w", [1, 2, 5, 8, 32])
