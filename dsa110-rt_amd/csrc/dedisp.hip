@@ -2,20 +2,35 @@
 // and chunk-maximum candidate extraction.  The DM x time x pixel cube is never
 // written: each work-group owns one pixel, each lane one DM trial.
 //
-// Work-group = 1 pixel x up to 256 DMs (lane -> DM, blockIdx.y -> DM group).
-// Per time tile of TT centres the work-group stages, for every subband s,
-// the window of steps any of its DMs reads into LDS (float32):
-//     tau in [c0 - W/2 - shmax_s, c0 + TT + W/2 - shmin_s)
-// Each lane then sweeps time in 64-centre chunks.  It keeps the exclusive
-// prefix sum P of its dedispersed series in a register window
+// Subband-pair partial sums.  For adjacent subbands a = 2g, b = 2g + 1 a DM
+// reads a[t - sh_a] + b[t - sh_b] = Q_delta[t - sh_a] with
+//     Q_delta[tau] = a[tau] + b[tau - delta],   delta = sh_b - sh_a,
+// and a pair has far fewer distinct deltas than the plan has DMs (12 for the
+// 256-trial 0-1500 plan over 16 subbands).  Per time tile the work-group
+// stages every subband's window of steps in LDS (ring dtype), builds one
+// float32 row per (pair, delta) trimmed to the span of steps its DMs read,
+// and each lane then sums n_sub / 2 rows per dedispersed sample instead of
+// n_sub subbands.  Plans whose pair rows do not fit the LDS budget (many
+// subbands with few DMs each) run unpaired: the lanes sum the subband
+// windows directly.
+//
+// Each lane sweeps time in 64-centre chunks and keeps the exclusive prefix
+// sum P of its dedispersed series in a register window
 //     Pw[x] = P[c - W/2 + x],  x in [0, 64 + 2 (W/2)]
-// computing every D(t) = sum_s win_s[t - shift[s][d]] exactly once, then
-//     B_w(c + i) = Pw[i + W/2 - w/2 + w] - Pw[i + W/2 - w/2]      (all widths)
-// The chunk maximum of each width's boxcar is turned into S/N with the
-// float64 state read at kernel start; a chunk whose best S/N exceeds the
-// threshold appends one candidate.  Block sums of complete centres are merged
-// into the float64 state at the end (Chan), so detection in a block uses only
-// earlier blocks' statistics.
+// so every D(t) is formed once; width-1 maxima and the block's statistics
+// come from the fresh D values, the other widths from prefix differences
+//     B_w(c + i) = Pw[i + W/2 - w/2 + w] - Pw[i + W/2 - w/2].
+// The chunk maximum of each width becomes an S/N with the float64 state read
+// at kernel start; a chunk whose best S/N exceeds the threshold appends one
+// candidate, located by recomputing the winning width's boxcars with the
+// same arithmetic (bit-identical values).  Block sums of complete samples are
+// merged into the float64 state at the end (Chan), so detection in a block
+// uses only earlier blocks' statistics.
+#include <algorithm>
+#include <climits>
+#include <type_traits>
+#include <vector>
+
 #include "dsx_internal.h"
 
 namespace dsx {
@@ -31,8 +46,11 @@ static_assert(sizeof(Cand) == 16, "candidate record must be 16 bytes");
 
 struct DedispParams {
     const void *ring;
-    const int32_t *shift;       // [s][d]
-    const int32_t *sub_info;    // [s][2] = (lds offset, shmax); shmin is 0 for DM 0
+    const int32_t *sub_info;    // [s][3] = (R offset, shmax_s, window length L_s)
+    const int32_t *rows;        // [r][6] = (Q offset, length, R index of a at e = 0,
+                                //           R index of b at e = 0, b's R bounds [lo, hi); empty
+                                //           bounds for a pair without b)
+    const int32_t *tbase;       // [g][d]: LDS element index of D(t)'s term g at t = tile start
     const int32_t *valid_from;  // [d]
     const double *n_prev;       // [d]
     const double *blk_count;    // [d]
@@ -41,61 +59,65 @@ struct DedispParams {
     Cand *cand;
     int32_t *n_cand;
     int64_t T0, c_lo;
-    int n_sub, n_pix, hist, row_len, n_dm, n_centres, tt, win_total, cand_cap;
+    int n_sub, n_pix, hist, row_len, n_dm, n_rows, n_terms, n_centres, tt, r_total, q_total,
+        cand_cap;
     float threshold;
     double min_count;
 };
 
 constexpr int DD_CHUNK = 64;
-
 constexpr int DD_THREADS = 256;   // DMs per work-group (grid.y covers the rest)
+constexpr int DD_MAX_ROWS = 4096;
 
-// D(t) of one DM from the staged windows; t_rel is relative to the tile's
-// first centre and the subbands are summed in the register path's order.
-__device__ __forceinline__ float window_d(const float *win, const int32_t *sub_info,
-                                          const int32_t *shift, int n_sub, int n_dm, int d,
-                                          int t_rel, int h2) {
-    float acc = 0.f;
-    for (int s = 0; s < n_sub; ++s)
-        acc += win[sub_info[2 * s] + sub_info[2 * s + 1] - shift[s * n_dm + d] + t_rel + h2];
+// D(t) of DM d from the term rows, in the main path's order;
+// t_rel = t - (first centre of the tile).
+template <typename E>
+__device__ __forceinline__ float terms_d(const E *X, const int32_t *tbase, int n_terms, int n_dm, int d,
+                                         int t_rel) {
+    float acc = (float)X[tbase[d] + t_rel];
+    for (int g = 1; g < n_terms; ++g) acc += (float)X[tbase[g * n_dm + d] + t_rel];
     return acc;
 }
 
-// Earliest centre i in [0, 64) of the chunk at tile offset cc whose width-2^k
-// boxcar is largest (running window sum).  Runs only for chunks that produce
-// a candidate; the reported S/N comes from the register maximum.
-__device__ __forceinline__ int rescan_chunk(const float *win, const int32_t *sub_info,
-                                            const int32_t *shift, int n_sub, int n_dm, int d, int k,
-                                            int cc, int h2) {
-    const int w = 1 << k;
-    const int lo = cc - (w >> 1);                  // first sample of centre cc's box
-    float run = 0.f;
-    for (int j = 0; j < w; ++j) run += window_d(win, sub_info, shift, n_sub, n_dm, d, lo + j, h2);
-    float best = run;
-    int bi = 0;
-    for (int i = 1; i < DD_CHUNK; ++i) {
-        run += window_d(win, sub_info, shift, n_sub, n_dm, d, lo + i + w - 1, h2) -
-               window_d(win, sub_info, shift, n_sub, n_dm, d, lo + i - 1, h2);
-        if (run > best) {
-            best = run;
-            bi = i;
+// Pw[off + j] (=|+=) sum over terms of X[base_g + j], j < n, term bases
+// prefetched one term ahead from global memory.
+template <int N, int OFF, typename E, int NP>
+__device__ __forceinline__ void sum_terms(float (&Pw)[NP], const E *X, const int32_t *tbase, int n_terms,
+                                          int n_dm, int d, int base0, int rel) {
+    int cur = base0;
+    for (int g = 0; g < n_terms; ++g) {
+        const int nxt = g + 1 < n_terms ? tbase[(g + 1) * n_dm + d] : 0;
+        const E *xg = X + cur + rel;
+        if (g == 0) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) Pw[OFF + j] = (float)xg[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < N; ++j) Pw[OFF + j] += (float)xg[j];
         }
+        cur = nxt;
     }
-    return bi;
 }
 
-template <typename T, int NW>
+template <typename T, int NW, bool PAIRED>
 __global__ __launch_bounds__(DD_THREADS) void dedisp_kernel(DedispParams P) {
     constexpr int WMAX = 1 << (NW - 1);
     constexpr int H2 = WMAX / 2;
     constexpr int WW = 2 * H2;
     constexpr int NP = DD_CHUNK + WW + 1;            // register prefix window
-    extern __shared__ float win[];
+    extern __shared__ float lds[];
+    float *Q = lds;                                   // [q_total] float32 pair rows
+    T *R = reinterpret_cast<T *>(lds + P.q_total);    // [r_total] subband windows
+    using E = typename std::conditional<PAIRED, float, T>::type;
+    const E *X = PAIRED ? reinterpret_cast<const E *>(Q) : reinterpret_cast<const E *>(R);
     const T *ring = reinterpret_cast<const T *>(P.ring);
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int d = blockIdx.y * DD_THREADS + tid;      // one lane per DM
     const bool active = d < P.n_dm;
+    const int64_t c_hi = P.c_lo + P.n_centres;
+
+    const int tb0 = active ? P.tbase[d] : 0;
 
     for (int p = blockIdx.x; p < P.n_pix; p += gridDim.x) {
         // ---- per-lane state from earlier blocks (float64)
@@ -118,93 +140,108 @@ __global__ __launch_bounds__(DD_THREADS) void dedisp_kernel(DedispParams P) {
             }
         }
         double s1 = 0.0, s2 = 0.0;
+        float m1_tail = -INFINITY;    // max of D over [c, c + H2) for the next chunk
         float Pw[NP];
 #pragma unroll
         for (int x = 0; x < NP; ++x) Pw[x] = 0.f;
 
         for (int c0 = 0; c0 < P.n_centres; c0 += P.tt) {
             const int64_t cabs0 = P.c_lo + c0;
-            const int tt = min(P.tt, P.n_centres - c0);
-            // ---- stage the windows of every subband (float32)
+            // ---- stage every subband's window (ring dtype)
             __syncthreads();
             for (int s = 0; s < P.n_sub; ++s) {
-                const int off = P.sub_info[2 * s];
-                const int shmax = P.sub_info[2 * s + 1];
-                const int len = tt + WW + shmax;
+                const int off = P.sub_info[3 * s];
+                const int shmax = P.sub_info[3 * s + 1];
+                const int len = P.sub_info[3 * s + 2];
                 const int64_t tau_lo = cabs0 - H2 - shmax;
                 const T *row = ring + ((int64_t)s * P.n_pix + p) * P.row_len;
                 for (int e = tid; e < len; e += nthr) {
                     const int64_t pos = P.hist + (tau_lo + e) - P.T0;
-                    float v = 0.f;
-                    if (pos >= 0 && pos < P.row_len) v = ld_as_float(row, pos);
-                    win[off + e] = v;
+                    T v = (T)0;
+                    if (pos >= 0 && pos < P.row_len) v = row[pos];
+                    R[off + e] = v;
+                }
+            }
+            __syncthreads();
+            // ---- one float32 row per (pair, delta): Q[e] = R[a0 + e] + R[b0 + e]
+            if (PAIRED) {
+                for (int r = 0; r < P.n_rows; ++r) {
+                    const int qoff = P.rows[6 * r], len = P.rows[6 * r + 1];
+                    const int a0 = P.rows[6 * r + 2], b0 = P.rows[6 * r + 3];
+                    const int blo = P.rows[6 * r + 4], bhi = P.rows[6 * r + 5];
+                    for (int e = tid; e < len; e += nthr) {
+                        float v = (float)R[a0 + e];
+                        const int ib = b0 + e;
+                        if (ib >= blo && ib < bhi) v += (float)R[ib];
+                        Q[qoff + e] = v;
+                    }
                 }
             }
             __syncthreads();
             if (!active) continue;
 
-            // index of step t for subband s in this tile: off_s + (t - tau_lo_s) - shift
-            //   = off_s + t - cabs0 + H2 + shmax_s - shift[s][d]
             if (c0 == 0) {
-                // prologue: Pw[0] = 0, Pw[x + 1] = Pw[x] + D(c_lo - H2 + x), x < WW
+                // prologue: D(c_lo - H2 + x), x < WW, into Pw[x + 1]
+                if (WW > 0) sum_terms<WW, 1>(Pw, X, P.tbase, P.n_terms, P.n_dm, d, tb0, -H2);
+                // samples t = c_lo + (x - H2) for x >= H2 belong to this block
+                float a1 = 0.f, a2 = 0.f, m = -INFINITY;
 #pragma unroll
-                for (int x = 0; x < NP; ++x) Pw[x] = 0.f;
-                for (int s = 0; s < P.n_sub; ++s) {
-                    const int base = P.sub_info[2 * s] + P.sub_info[2 * s + 1] - P.shift[s * P.n_dm + d];
-                    // t = cabs0 - H2 + x  ->  index = base + x
-#pragma unroll
-                    for (int x = 0; x < WW; ++x) Pw[x + 1] += win[base + x];
+                for (int x = H2; x < WW; ++x) {
+                    const float v = Pw[x + 1];
+                    const int64_t t = P.c_lo + (x - H2);
+                    const bool ok = t >= vf && t < c_hi;
+                    a1 += ok ? v : 0.f;
+                    a2 = fmaf(ok ? v : 0.f, v, a2);
+                    m = fmaxf(m, v);
                 }
+                s1 += (double)a1;
+                s2 += (double)a2;
+                m1_tail = m;
+                Pw[0] = 0.f;
 #pragma unroll
                 for (int x = 0; x < WW; ++x) Pw[x + 1] += Pw[x];
             }
 
-            for (int cc = 0; cc < tt; cc += DD_CHUNK) {
+            for (int cc = 0; cc < P.tt; cc += DD_CHUNK) {
                 const int64_t c = cabs0 + cc;
-                // ---- D(c + H2 + i), i < 64, accumulated over subbands into the
-                //      window slots, then turned into prefix sums in place
+                // ---- D(c + H2 + i), i < 64, summed over the term rows into the window slots
+                sum_terms<DD_CHUNK, WW + 1>(Pw, X, P.tbase, P.n_terms, P.n_dm, d, tb0, cc + H2);
+                // ---- statistics of this block's complete samples, width-1 maxima
+                const int64_t t1 = c + H2;
+                float a1 = 0.f, a2 = 0.f;
+                const bool full = t1 >= vf && t1 + DD_CHUNK <= c_hi;
+                if (full) {
 #pragma unroll
-                for (int i = 0; i < DD_CHUNK; ++i) Pw[WW + 1 + i] = 0.f;
-                for (int s = 0; s < P.n_sub; ++s) {
-                    // t = c + H2 + i -> index = off + (c - cabs0) + 2 H2 + shmax - shift + i
-                    const int base = P.sub_info[2 * s] + P.sub_info[2 * s + 1] - P.shift[s * P.n_dm + d] +
-                                     cc + WW;
-#pragma unroll
-                    for (int i = 0; i < DD_CHUNK; ++i) Pw[WW + 1 + i] += win[base + i];
-                }
-#pragma unroll
-                for (int i = 0; i < DD_CHUNK; ++i) Pw[WW + 1 + i] += Pw[WW + i];
-
-                // ---- statistics of complete centres (D(c + i) = B_1)
-                if (c >= vf) {
-                    float a1 = 0.f, a2 = 0.f;
+                    for (int i = 0; i < DD_CHUNK; ++i) a2 = fmaf(Pw[WW + 1 + i], Pw[WW + 1 + i], a2);
+                } else {
 #pragma unroll
                     for (int i = 0; i < DD_CHUNK; ++i) {
-                        const float v = Pw[i + H2 + 1] - Pw[i + H2];
-                        a1 += v;
-                        a2 = fmaf(v, v, a2);
-                    }
-                    s1 += (double)a1;
-                    s2 += (double)a2;
-                } else if (c + DD_CHUNK > vf) {
-                    const int i_first = (int)(vf - c);    // first complete centre of the chunk
-                    float a1 = 0.f, a2 = 0.f;
-#pragma unroll
-                    for (int i = 0; i < DD_CHUNK; ++i) {
-                        const float v = Pw[i + H2 + 1] - Pw[i + H2];
-                        const bool ok = i >= i_first;
+                        const float v = Pw[WW + 1 + i];
+                        const bool ok = t1 + i >= vf && t1 + i < c_hi;
                         a1 += ok ? v : 0.f;
                         a2 = fmaf(ok ? v : 0.f, v, a2);
                     }
-                    s1 += (double)a1;
-                    s2 += (double)a2;
                 }
+                float mA = -INFINITY, mB = -INFINITY;
+#pragma unroll
+                for (int i = 0; i < DD_CHUNK - H2; ++i) mA = fmaxf(mA, Pw[WW + 1 + i]);
+#pragma unroll
+                for (int i = DD_CHUNK - H2; i < DD_CHUNK; ++i) mB = fmaxf(mB, Pw[WW + 1 + i]);
+                const float m1 = fmaxf(m1_tail, mA);
+                m1_tail = mB;
+                // ---- prefix sums in place
+#pragma unroll
+                for (int i = 0; i < DD_CHUNK; ++i) Pw[WW + 1 + i] += Pw[WW + i];
+                if (full) a1 = Pw[WW + DD_CHUNK] - Pw[WW];
+                s1 += (double)a1;
+                s2 += (double)a2;
 
                 // ---- boxcar chunk maxima and candidate
                 if (score && c - H2 >= vf) {
                     float mb[NW];
+                    mb[0] = m1;
 #pragma unroll
-                    for (int k = 0; k < NW; ++k) {
+                    for (int k = 1; k < NW; ++k) {
                         const int w = 1 << k;
                         const int lo = H2 - (w >> 1);
                         float m = -INFINITY;
@@ -223,10 +260,34 @@ __global__ __launch_bounds__(DD_THREADS) void dedisp_kernel(DedispParams P) {
                         }
                     }
                     if (best > P.threshold) {
-                        // rare path: locate the earliest centre of the winning
-                        // width by re-summing D from the LDS window
-                        const int bi = rescan_chunk(win, P.sub_info, P.shift, P.n_sub, P.n_dm, d, bk,
-                                                    cc, H2);
+                        // rare path: earliest centre of the winning width, from
+                        // values computed exactly as the maxima were
+                        int bi = 0;
+                        if (bk == 0) {
+                            for (int i = 0; i < DD_CHUNK; ++i) {
+                                if (terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc + i) == mb[0]) {
+                                    bi = i;
+                                    break;
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int k = 1; k < NW; ++k) {
+                                if (k == bk) {
+                                    const int w = 1 << k;
+                                    const int lo = H2 - (w >> 1);
+#pragma unroll
+                                    for (int i = DD_CHUNK - 1; i >= 0; --i) {
+                                        // the empty asm keeps the compiler from reusing the
+                                        // maxima phase's differences (which would keep all
+                                        // of them live until here); the values are the same
+                                        float vh = Pw[i + lo + w], vl = Pw[i + lo];
+                                        asm volatile("" : "+v"(vh), "+v"(vl));
+                                        if (vh - vl == mb[k]) bi = i;
+                                    }
+                                }
+                            }
+                        }
                         const int slot = atomicAdd(P.n_cand, 1);
                         if (slot < P.cand_cap) {
                             Cand cd;
@@ -259,72 +320,225 @@ __global__ __launch_bounds__(DD_THREADS) void dedisp_kernel(DedispParams P) {
     }
 }
 
-template <typename T, int NW>
+template <typename T, int NW, bool PAIRED>
 static int launch_dedisp(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
-    (void)hipFuncSetAttribute((const void *)dedisp_kernel<T, NW>,
+    (void)hipFuncSetAttribute((const void *)dedisp_kernel<T, NW, PAIRED>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    dedisp_kernel<T, NW><<<grid, threads, lds, s>>>(P);
+    dedisp_kernel<T, NW, PAIRED><<<grid, threads, lds, s>>>(P);
     return check_launch("dsx_dedisp_detect");
 }
 
-template <typename T>
+template <typename T, bool PAIRED>
 static int dispatch_nw(int nw, const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
     switch (nw) {
-        case 1: return launch_dedisp<T, 1>(P, threads, lds, grid, s);
-        case 2: return launch_dedisp<T, 2>(P, threads, lds, grid, s);
-        case 3: return launch_dedisp<T, 3>(P, threads, lds, grid, s);
-        case 4: return launch_dedisp<T, 4>(P, threads, lds, grid, s);
-        case 5: return launch_dedisp<T, 5>(P, threads, lds, grid, s);
-        case 6: return launch_dedisp<T, 6>(P, threads, lds, grid, s);
-        case 7: return launch_dedisp<T, 7>(P, threads, lds, grid, s);
+        case 1: return launch_dedisp<T, 1, PAIRED>(P, threads, lds, grid, s);
+        case 2: return launch_dedisp<T, 2, PAIRED>(P, threads, lds, grid, s);
+        case 3: return launch_dedisp<T, 3, PAIRED>(P, threads, lds, grid, s);
+        case 4: return launch_dedisp<T, 4, PAIRED>(P, threads, lds, grid, s);
+        case 5: return launch_dedisp<T, 5, PAIRED>(P, threads, lds, grid, s);
+        case 6: return launch_dedisp<T, 6, PAIRED>(P, threads, lds, grid, s);
+        case 7: return launch_dedisp<T, 7, PAIRED>(P, threads, lds, grid, s);
         default: return set_error(DSX_ENOTSUP, "dsx_dedisp_detect: n_widths=%d not in [1, 7]", nw);
     }
+}
+
+// Pair rows of a plan: per pair g the sorted distinct deltas and, per delta,
+// the range [lo, hi] of sh_a over the DMs that use it.
+struct PairRows {
+    std::vector<int> a, b, delta, lo, hi;   // one entry per row
+    std::vector<int> first;                  // first row of pair g, size n_pair + 1
+};
+
+static PairRows pair_rows(const int32_t *sh, int n_sub, int n_dm) {
+    PairRows pr;
+    const int n_pair = (n_sub + 1) / 2;
+    pr.first.push_back(0);
+    for (int g = 0; g < n_pair; ++g) {
+        const int a = 2 * g, b = 2 * g + 1 < n_sub ? 2 * g + 1 : -1;
+        std::vector<int> dl;
+        for (int d = 0; d < n_dm; ++d)
+            dl.push_back(b >= 0 ? sh[(int64_t)b * n_dm + d] - sh[(int64_t)a * n_dm + d] : 0);
+        std::vector<int> u = dl;
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        for (int v : u) {
+            int lo = INT32_MAX, hi = INT32_MIN;
+            for (int d = 0; d < n_dm; ++d)
+                if (dl[d] == v) {
+                    lo = std::min(lo, (int)sh[(int64_t)a * n_dm + d]);
+                    hi = std::max(hi, (int)sh[(int64_t)a * n_dm + d]);
+                }
+            pr.a.push_back(a);
+            pr.b.push_back(b);
+            pr.delta.push_back(v);
+            pr.lo.push_back(lo);
+            pr.hi.push_back(hi);
+        }
+        pr.first.push_back((int)pr.a.size());
+    }
+    return pr;
 }
 
 }  // namespace dsx
 
 using namespace dsx;
 
-extern "C" int dsx_dedisp_tile(int n_sub, const int32_t *host_shmax, int n_widths, int n_centres,
-                               int lds_budget) {
-    // Largest tile (multiple of 64, <= n_centres) whose windows fit the LDS budget.
-    int64_t span = 0;
-    for (int s = 0; s < n_sub; ++s) span += host_shmax[s];
+extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, int n_widths,
+                               int n_centres, int ring_dtype, int lds_budget, int mode,
+                               int32_t *sub_info, int32_t *rows, int rows_cap, int32_t *tbase,
+                               int32_t *info) {
+    DSX_REQUIRE(host_shift && sub_info && rows && tbase && info, "dsx_dedisp_plan: null pointer");
+    DSX_REQUIRE(n_sub > 0 && n_sub <= 1024, "dsx_dedisp_plan: n_sub=%d not in [1, 1024]", n_sub);
+    DSX_REQUIRE(n_dm > 0 && n_dm <= 32767, "dsx_dedisp_plan: n_dm=%d", n_dm);
+    DSX_REQUIRE(n_widths >= 1 && n_widths <= 7, "dsx_dedisp_plan: n_widths=%d", n_widths);
+    DSX_REQUIRE(n_centres > 0 && n_centres % DD_CHUNK == 0,
+                "dsx_dedisp_plan: n_centres must be a positive multiple of 64");
+    DSX_REQUIRE(ring_dtype == DSX_F16 || ring_dtype == DSX_F32, "dsx_dedisp_plan: ring_dtype");
+    DSX_REQUIRE(mode >= 0 && mode <= 2, "dsx_dedisp_plan: mode=%d not in {0, 1, 2}", mode);
+    const int64_t esize = ring_dtype == DSX_F16 ? 2 : 4;
     const int ww = n_widths > 1 ? (1 << (n_widths - 1)) : 0;
-    int best = 0;
-    for (int tt = 64; tt <= n_centres && tt <= 2048; tt += 64) {
-        const int64_t floats = (int64_t)n_sub * (tt + ww) + span;
-        if (floats * 4 <= lds_budget && n_centres % tt == 0) best = tt;
+    const int h2 = ww / 2;
+    std::vector<int> shmax(n_sub), shmin(n_sub);
+    for (int s = 0; s < n_sub; ++s) {
+        int hi = host_shift[(int64_t)s * n_dm], lo = hi;
+        for (int d = 1; d < n_dm; ++d) {
+            hi = std::max(hi, (int)host_shift[(int64_t)s * n_dm + d]);
+            lo = std::min(lo, (int)host_shift[(int64_t)s * n_dm + d]);
+        }
+        shmax[s] = hi;
+        shmin[s] = lo;
     }
-    return best;
+    const PairRows pr = pair_rows(host_shift, n_sub, n_dm);
+    const int n_prow = (int)pr.a.size();
+    auto r_elems = [&](int tt) {
+        int64_t r = 0;
+        for (int s = 0; s < n_sub; ++s) r += tt + ww + shmax[s] - shmin[s];
+        return r;
+    };
+    auto q_elems = [&](int tt) {
+        int64_t q = 0;
+        for (int r = 0; r < n_prow; ++r) q += tt + ww + pr.hi[r] - pr.lo[r];
+        return q;
+    };
+    auto best_tile = [&](bool paired) {
+        int best = 0;
+        for (int t = DD_CHUNK; t <= n_centres && t <= 2048; t += DD_CHUNK) {
+            const int64_t bytes = (paired ? q_elems(t) * 4 : 0) + r_elems(t) * esize;
+            if (n_centres % t == 0 && bytes <= lds_budget) best = t;
+        }
+        return best;
+    };
+    bool paired = false;
+    int tt = 0;
+    if (mode != 2 && n_prow <= DD_MAX_ROWS) {
+        tt = best_tile(true);
+        paired = tt > 0;
+    }
+    if (!paired) {
+        if (mode == 1)
+            return set_error(DSX_ERANGE, "dsx_dedisp_plan: pair rows need %lld B of LDS > %d",
+                             (long long)(q_elems(DD_CHUNK) * 4 + r_elems(DD_CHUNK) * esize), lds_budget);
+        tt = best_tile(false);
+    }
+    if (tt == 0)
+        return set_error(DSX_ERANGE, "dsx_dedisp_plan: the 64-centre tile needs %lld B of LDS > %d",
+                         (long long)(r_elems(DD_CHUNK) * esize), lds_budget);
+    const int n_rows = paired ? n_prow : 0;
+    if (n_rows > rows_cap)
+        return set_error(DSX_ERANGE, "dsx_dedisp_plan: %d pair rows > rows_cap %d", n_rows, rows_cap);
+    // subband windows: R index of step tau for subband s = off_s + tau - (tile start - W/2 - shmax_s)
+    int r_total = 0;
+    std::vector<int> roff(n_sub);
+    for (int s = 0; s < n_sub; ++s) {
+        const int len = tt + ww + shmax[s] - shmin[s];
+        roff[s] = r_total;
+        sub_info[3 * s] = r_total;
+        sub_info[3 * s + 1] = shmax[s];
+        sub_info[3 * s + 2] = len;
+        r_total += len;
+    }
+    int q_total = 0;
+    if (paired) {
+        // row r covers tau in [tile start - W/2 - hi_r, + len_r) of its pair's DMs
+        std::vector<int> qoff(n_prow);
+        for (int r = 0; r < n_prow; ++r) {
+            const int a = pr.a[r], b = pr.b[r];
+            const int len = tt + ww + pr.hi[r] - pr.lo[r];
+            qoff[r] = q_total;
+            rows[6 * r] = q_total;
+            rows[6 * r + 1] = len;
+            rows[6 * r + 2] = roff[a] + shmax[a] - pr.hi[r];
+            if (b >= 0) {
+                rows[6 * r + 3] = roff[b] + shmax[b] - pr.hi[r] - pr.delta[r];
+                rows[6 * r + 4] = roff[b];
+                rows[6 * r + 5] = roff[b] + tt + ww + shmax[b] - shmin[b];
+            } else {
+                rows[6 * r + 3] = 0;
+                rows[6 * r + 4] = 0;
+                rows[6 * r + 5] = 0;
+            }
+            q_total += len;
+        }
+        const int n_pair = (n_sub + 1) / 2;
+        for (int g = 0; g < n_pair; ++g) {
+            const int a = 2 * g, b = 2 * g + 1 < n_sub ? 2 * g + 1 : -1;
+            for (int d = 0; d < n_dm; ++d) {
+                const int sa = host_shift[(int64_t)a * n_dm + d];
+                const int dl = b >= 0 ? host_shift[(int64_t)b * n_dm + d] - sa : 0;
+                int r = pr.first[g];
+                while (pr.delta[r] != dl) ++r;
+                tbase[(int64_t)g * n_dm + d] = qoff[r] + h2 + pr.hi[r] - sa;
+            }
+        }
+    } else {
+        for (int s = 0; s < n_sub; ++s)
+            for (int d = 0; d < n_dm; ++d)
+                tbase[(int64_t)s * n_dm + d] = roff[s] + h2 + shmax[s] - host_shift[(int64_t)s * n_dm + d];
+    }
+    info[0] = tt;
+    info[1] = n_rows;
+    info[2] = paired ? (n_sub + 1) / 2 : n_sub;   // terms per dedispersed sample
+    info[3] = r_total;
+    info[4] = q_total;
+    info[5] = paired ? 1 : 0;
+    return DSX_OK;
 }
 
 extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, int n_pix, int hist,
-                                 int ring_len, int64_t T0, const int32_t *shift, int n_dm,
-                                 const int32_t *sub_info, const int32_t *valid_from,
-                                 const double *n_prev, const double *blk_count, double *mean,
-                                 double *m2, int64_t c_lo, int n_centres, int n_widths, int tile,
-                                 int win_total, float threshold, double min_count, void *cand,
-                                 int cand_cap, int32_t *n_cand, dsx_stream_t stream) {
-    DSX_REQUIRE(ring && shift && sub_info && valid_from && n_prev && blk_count && mean && m2 &&
-                    cand && n_cand,
+                                 int ring_len, int64_t T0, const int32_t *sub_info,
+                                 const int32_t *rows, int n_rows, const int32_t *tbase, int n_terms,
+                                 int n_dm, const int32_t *valid_from, const double *n_prev,
+                                 const double *blk_count, double *mean, double *m2, int64_t c_lo,
+                                 int n_centres, int n_widths, int tile, int r_total, int q_total,
+                                 float threshold, double min_count, void *cand, int cand_cap,
+                                 int32_t *n_cand, dsx_stream_t stream) {
+    DSX_REQUIRE(ring && sub_info && tbase && valid_from && n_prev && blk_count && mean && m2 && cand &&
+                    n_cand,
                 "dsx_dedisp_detect: null pointer");
     DSX_REQUIRE(ring_dtype == DSX_F16 || ring_dtype == DSX_F32, "dsx_dedisp_detect: ring_dtype");
     DSX_REQUIRE(n_sub > 0 && n_pix > 0 && n_dm > 0 && hist >= 0 && ring_len > hist,
                 "dsx_dedisp_detect: bad shape");
     DSX_REQUIRE(n_dm <= 32767, "dsx_dedisp_detect: n_dm=%d > 32767", n_dm);
+    const bool paired = n_rows > 0;
+    DSX_REQUIRE(paired ? (rows && n_terms == (n_sub + 1) / 2 && q_total > 0 && n_rows <= DD_MAX_ROWS)
+                       : (n_terms == n_sub && q_total == 0),
+                "dsx_dedisp_detect: n_rows=%d / n_terms=%d / q_total=%d do not describe a plan for n_sub=%d",
+                n_rows, n_terms, q_total, n_sub);
     DSX_REQUIRE(n_centres > 0 && n_centres % DD_CHUNK == 0,
                 "dsx_dedisp_detect: n_centres must be a positive multiple of 64");
     DSX_REQUIRE(tile > 0 && tile % DD_CHUNK == 0 && n_centres % tile == 0,
                 "dsx_dedisp_detect: tile must be a multiple of 64 dividing n_centres");
-    DSX_REQUIRE(cand_cap >= 0, "dsx_dedisp_detect: cand_cap");
-    const size_t lds = (size_t)win_total * sizeof(float);
+    DSX_REQUIRE(n_widths >= 1 && n_widths <= 7, "dsx_dedisp_detect: n_widths=%d", n_widths);
+    DSX_REQUIRE(cand_cap >= 0 && r_total > 0, "dsx_dedisp_detect: sizes");
+    const size_t esize = ring_dtype == DSX_F16 ? 2 : 4;
+    const size_t lds = (size_t)q_total * 4 + (size_t)r_total * esize;
     if (lds > 160 * 1024)
         return set_error(DSX_ERANGE, "dsx_dedisp_detect: %zu B of LDS for the time windows", lds);
     DedispParams P{};
     P.ring = ring;
-    P.shift = shift;
     P.sub_info = sub_info;
+    P.rows = rows;
+    P.tbase = tbase;
     P.valid_from = valid_from;
     P.n_prev = n_prev;
     P.blk_count = blk_count;
@@ -339,15 +553,21 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
     P.hist = hist;
     P.row_len = ring_len;
     P.n_dm = n_dm;
+    P.n_rows = n_rows;
+    P.n_terms = n_terms;
     P.n_centres = n_centres;
     P.tt = tile;
-    P.win_total = win_total;
+    P.r_total = r_total;
+    P.q_total = q_total;
     P.cand_cap = cand_cap;
     P.threshold = threshold;
     P.min_count = min_count;
     const int threads = std::min(DD_THREADS, ((n_dm + 63) / 64) * 64);
     const dim3 grid((unsigned)std::min(n_pix, 1 << 20), (unsigned)((n_dm + DD_THREADS - 1) / DD_THREADS));
     hipStream_t s = as_stream(stream);
-    if (ring_dtype == DSX_F16) return dispatch_nw<_Float16>(n_widths, P, threads, lds, grid, s);
-    return dispatch_nw<float>(n_widths, P, threads, lds, grid, s);
+    if (ring_dtype == DSX_F16)
+        return paired ? dispatch_nw<_Float16, true>(n_widths, P, threads, lds, grid, s)
+                      : dispatch_nw<_Float16, false>(n_widths, P, threads, lds, grid, s);
+    return paired ? dispatch_nw<float, true>(n_widths, P, threads, lds, grid, s)
+                  : dispatch_nw<float, false>(n_widths, P, threads, lds, grid, s);
 }

@@ -184,6 +184,52 @@ def node_shift_table(dm_pc_cm3: np.ndarray, nu_subband_ref_GHz: np.ndarray, nu_b
     return subband_shift_table(dm_pc_cm3, nu_subband_ref_GHz, nu_bot_GHz, t_int_search_us)
 
 
+@dataclass(frozen=True)
+class DedispPlan:
+    """LDS layout of ``dsx_dedisp_detect`` for one DM plan and block length
+    (``dsx_dedisp_plan``): host arrays, copied to the device by the caller."""
+    tile: int
+    n_rows: int
+    n_terms: int
+    r_total: int
+    q_total: int
+    paired: bool
+    sub_info: np.ndarray      # [n_sub, 3] int32
+    rows: np.ndarray          # [n_rows, 6] int32
+    tbase: np.ndarray         # [n_terms, n_dm] int32
+
+    @property
+    def lds_bytes(self) -> int:
+        return self.q_total * 4 + self.r_total * self._esize
+
+    _esize: int = 2
+
+
+def plan_dedisp(shift_sd: np.ndarray, *, n_widths: int, n_centres: int, ring_dtype: torch.dtype,
+                lds_budget: int, mode: int = 0) -> DedispPlan:
+    """Run ``dsx_dedisp_plan`` (host code, no GPU) for ``shift_sd`` = [n_sub, n_dm].
+    ``mode`` 0 = pair rows when they fit, 1 = pair rows only, 2 = unpaired."""
+    sh = np.ascontiguousarray(shift_sd, dtype=np.int32)
+    if sh.ndim != 2:
+        raise ValueError("shift_sd must be [n_sub, n_dm]")
+    n_sub, n_dm = sh.shape
+    code = _lib.DSX_F16 if ring_dtype == torch.float16 else _lib.DSX_F32
+    n_pair = (n_sub + 1) // 2
+    rows_cap = int(sum(len(np.unique(sh[min(2 * g + 1, n_sub - 1)] - sh[2 * g])) for g in range(n_pair)))
+    sub_info = np.zeros((n_sub, 3), np.int32)
+    rows = np.zeros((max(rows_cap, 1), 6), np.int32)
+    tbase = np.zeros((max(n_sub, n_pair), n_dm), np.int32)
+    info = np.zeros(6, np.int32)
+    P = _lib._P
+    _lib.call("dsx_dedisp_plan", sh.ctypes.data_as(P), n_sub, n_dm, n_widths, n_centres, code, int(lds_budget),
+              int(mode), sub_info.ctypes.data_as(P), rows.ctypes.data_as(P), rows_cap, tbase.ctypes.data_as(P),
+              info.ctypes.data_as(P))
+    tile, n_rows, n_terms, r_total, q_total, paired = (int(x) for x in info)
+    return DedispPlan(tile=tile, n_rows=n_rows, n_terms=n_terms, r_total=r_total, q_total=q_total,
+                      paired=bool(paired), sub_info=sub_info, rows=rows[:n_rows].copy(),
+                      tbase=tbase[:n_terms].copy(), _esize=2 if code == _lib.DSX_F16 else 4)
+
+
 def combine_chgroups(*, per_chgroup_streams: Mapping[int, torch.Tensor], time_shift_per_chgroup,
                      t_window: Tuple[int, int], n_grid: Optional[int] = None,
                      n_chgroup: Optional[int] = None) -> torch.Tensor:

@@ -24,11 +24,13 @@ import torch
 
 from . import _lib
 from .config import DETECT_CHUNK_STEPS, DETECT_DELAY_STEPS, NodeConfig
+from .dedisp import plan_dedisp
 from .detector import CAND_DTYPE, Candidate, decode_chunk_maxima
 from .imager import mask_npad
 from .pattern import build_cell_map, build_imager_taps
 
-LDS_BUDGET_DEDISP = 64 * 1024
+#: LDS per dedispersion work-group: two 256-lane work-groups per CU.
+LDS_BUDGET_DEDISP = 80 * 1024
 
 
 @dataclass
@@ -47,7 +49,8 @@ class SearchNode:
     def __init__(self, cfg: NodeConfig, antpos_e: np.ndarray, antpos_n: np.ndarray, *,
                  device: str = "cuda", ring_dtype: torch.dtype = torch.float16,
                  max_block_steps: Optional[int] = None, cand_cap: int = 1 << 20,
-                 stream: Optional[torch.cuda.Stream] = None):
+                 stream: Optional[torch.cuda.Stream] = None, lds_budget: int = LDS_BUDGET_DEDISP,
+                 dedisp_mode: int = 0):
         cfg.validate()
         if ring_dtype not in (torch.float16, torch.float32):
             raise TypeError("ring_dtype must be float16 or float32")
@@ -98,7 +101,9 @@ class SearchNode:
         self.cand_cap = int(cand_cap)
         self.cand = torch.empty((self.cand_cap, 4), dtype=torch.int32, device=self.dev)
         self.n_cand = torch.zeros(1, dtype=torch.int32, device=self.dev)
-        self.tiles: Dict[int, tuple] = {}
+        self.lds_budget = int(lds_budget)
+        self.dedisp_mode = int(dedisp_mode)
+        self.plans: Dict[int, tuple] = {}
         self.T0 = 0
         self.prev_len = 0
         self.marks: Optional[Dict[str, list]] = None     # stage -> [start, end] events when timing
@@ -120,23 +125,20 @@ class SearchNode:
         return {n: e[0].elapsed_time(e[1]) for n, e in (self.marks or {}).items()}
 
     # ------------------------------------------------------------------ helpers
-    def _tile_for(self, n_centres: int):
-        if n_centres not in self.tiles:
-            shmax = np.ascontiguousarray(self.shmax, dtype=np.int32)
-            tile = _lib.call("dsx_dedisp_tile", self.cfg.n_sub, shmax.ctypes.data_as(_lib._P),
-                             self.n_widths, n_centres, LDS_BUDGET_DEDISP)
-            if tile <= 0:
-                tile = _lib.call("dsx_dedisp_tile", self.cfg.n_sub, shmax.ctypes.data_as(_lib._P),
-                                 self.n_widths, n_centres, 160 * 1024)
-            if tile <= 0:
-                raise RuntimeError("DM plan too wide for the LDS windows of dsx_dedisp_detect")
-            ww = 2 * (self.w_max // 2)
-            lens = tile + ww + self.shmax.astype(np.int64)
-            offs = np.concatenate([[0], np.cumsum(lens)[:-1]])
-            info = np.stack([offs, self.shmax], axis=1).astype(np.int32)
-            self.tiles[n_centres] = (tile, int(lens.sum()),
-                                     torch.from_numpy(np.ascontiguousarray(info)).to(self.dev))
-        return self.tiles[n_centres]
+    def _plan_for(self, n_centres: int):
+        """Device copy of the dedispersion LDS plan for blocks of ``n_centres`` steps."""
+        if n_centres not in self.plans:
+            kw = dict(n_widths=self.n_widths, n_centres=n_centres, ring_dtype=self.ring_dtype,
+                      mode=self.dedisp_mode)
+            try:
+                plan = plan_dedisp(self.shifts_host.T, lds_budget=self.lds_budget, **kw)
+            except RuntimeError:
+                # wide plans: one work-group per CU with the whole LDS
+                plan = plan_dedisp(self.shifts_host.T, lds_budget=160 * 1024, **kw)
+            dev = {k: torch.from_numpy(np.ascontiguousarray(getattr(plan, k))).to(self.dev)
+                   for k in ("sub_info", "rows", "tbase")}
+            self.plans[n_centres] = (plan, dev)
+        return self.plans[n_centres]
 
     def _block_counts(self, c_lo: int, L: int) -> np.ndarray:
         start = np.maximum(self.valid_from_host, c_lo)
@@ -187,17 +189,19 @@ class SearchNode:
         cfg = self.cfg
         T0 = self.T0
         c_lo = T0 - DETECT_DELAY_STEPS
-        tile, win_total, info = self._tile_for(n_steps)
+        plan, pd = self._plan_for(n_steps)
         counts = self._block_counts(c_lo, n_steps)
         self.blk_count.copy_(torch.from_numpy(counts), non_blocking=False)
         self.n_cand.zero_()
         self._mark("dedisp", 0)
         _lib.call("dsx_dedisp_detect", _lib.ptr(self.rings[self.cur]),
                   _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
-                  cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(self.shift), cfg.n_dm,
-                  _lib.ptr(info), _lib.ptr(self.valid_from), _lib.ptr(self.n_prev),
+                  cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(pd["sub_info"]),
+                  _lib.ptr(pd["rows"]) if plan.n_rows else None, plan.n_rows, _lib.ptr(pd["tbase"]),
+                  plan.n_terms, cfg.n_dm, _lib.ptr(self.valid_from), _lib.ptr(self.n_prev),
                   _lib.ptr(self.blk_count), _lib.ptr(self.mean), _lib.ptr(self.m2), c_lo, n_steps,
-                  self.n_widths, tile, win_total, float(cfg.threshold), float(cfg.min_stat_count),
+                  self.n_widths, plan.tile, plan.r_total, plan.q_total, float(cfg.threshold),
+                  float(cfg.min_stat_count),
                   _lib.ptr(self.cand), self.cand_cap, _lib.ptr(self.n_cand), _lib.stream(self.stream))
         self._mark("dedisp", 1)
         n = int(self.n_cand.item())

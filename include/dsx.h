@@ -124,7 +124,25 @@ int dsx_ring_advance(const void *ring_old, void *ring_new, const void *img, int 
                      dsx_stream_t stream);
 
 /* ------------------------------------------------ dedispersion + detector
- * One block of centres [c_lo, c_lo + n_centres) (multiple of 64) for every
+ * Layout of the LDS time windows for a DM plan (host function, no GPU work).
+ * host_shift is [s][d] (host memory).  Picks the largest time tile (multiple
+ * of 64 dividing n_centres, <= 2048) whose windows fit lds_budget bytes.
+ * mode 0: pair rows when they fit, else unpaired; 1: pair rows only;
+ * 2: unpaired only.  Outputs (host memory):
+ *   sub_info [n_sub][3] = (window offset, max_d shift[s][d], window length)
+ *   rows     [rows_cap][6] pair rows (Q offset, length, R index of subband a
+ *            at e = 0, R index of subband b at e = 0, b's window [lo, hi))
+ *   tbase    [n_terms][n_dm] LDS index of each term of D(t) at t = tile start
+ *   info[6]  = (tile, n_rows, n_terms, r_total, q_total, paired)
+ * Pair rows: for subbands a = 2g, b = 2g + 1 and each distinct
+ * delta = shift[b][d] - shift[a][d], Q[tau] = ring_a[tau] + ring_b[tau - delta]
+ * over the span of steps the DMs with that delta read; D(t) is then the sum
+ * of n_sub / 2 rows instead of n_sub subbands. */
+int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, int n_widths, int n_centres,
+                    int ring_dtype, int lds_budget, int mode, int32_t *sub_info, int32_t *rows,
+                    int rows_cap, int32_t *tbase, int32_t *info);
+
+/* One block of centres [c_lo, c_lo + n_centres) (multiple of 64) for every
  * DM and pixel of the time-major ring (n_sub, n_pix, ring_len), where
  * position hist + tau - T0 holds absolute step tau:
  *   D[d][t][p] = sum_s ring[s][p][hist + (t - shift[s][d]) - T0]
@@ -136,31 +154,25 @@ int dsx_ring_advance(const void *ring_old, void *ring_new, const void *img, int 
  * then earliest centre) is appended to cand (16-byte records
  * {int32 t, int32 p, int16 d, int16 k, float snr}) when snr > threshold,
  * counted by *n_cand (device int32; entries past cand_cap are dropped but
- * still counted).  Complete centres (t >= valid_from[d]) feed the block's
- * sum / sum of squares, which are merged (Chan) into the float64 running
- * mean / m2 ([p][d] layout) with blk_count[d] samples.  shift is [s][d]
- * with shift[s][0] = 0.  The work-group stages per-subband windows of
- * `tile` centres in LDS: sub_info[s] = (offset, max_d shift[s][d]) with
- * offset_s = sum_{s' < s} (tile + 2 (w_max / 2) + max shift_s') and
- * win_total the sum over all s; dsx_dedisp_tile() picks `tile`.
- * ring_dtype DSX_F16 or DSX_F32; hist >= 64 + w_max/2 + max shift.
+ * still counted).  Samples t in [c_lo, c_lo + n_centres) with
+ * t >= valid_from[d] feed the block's sum / sum of squares, merged (Chan)
+ * into the float64 running mean / m2 ([p][d] layout) with blk_count[d]
+ * samples.  sub_info, rows, tbase, n_rows, n_terms, tile, r_total and
+ * q_total come from dsx_dedisp_plan (copied to device memory; rows may be
+ * NULL when n_rows == 0).  ring_dtype DSX_F16 or DSX_F32;
+ * hist >= 64 + w_max/2 + max shift.
  * Replaces combine_chgroups + boxcar_via_cumsum + multi_boxcar_argmax +
  * the Layer-1/Layer-2 normalisation, src/dsart/fine_dm/combiner.py:548-655,
  * src/dsart/detector/forward.py:163-504,1264-1854,
  * src/dsart/detector/triton_boxcar.py:193-403,
  * src/dsart/noise_norm/layer2.py:313-452. */
 int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, int n_pix, int hist,
-                      int ring_len, int64_t T0, const int32_t *shift, int n_dm,
-                      const int32_t *sub_info, const int32_t *valid_from, const double *n_prev,
-                      const double *blk_count, double *mean, double *m2, int64_t c_lo,
-                      int n_centres, int n_widths, int tile, int win_total, float threshold,
-                      double min_count, void *cand, int cand_cap, int32_t *n_cand,
-                      dsx_stream_t stream);
-
-/* Largest time tile (multiple of 64 dividing n_centres, <= 2048) whose LDS
- * windows fit lds_budget bytes; 0 if none.  host_shmax[s] = max_d shift[s][d]. */
-int dsx_dedisp_tile(int n_sub, const int32_t *host_shmax, int n_widths, int n_centres,
-                    int lds_budget);
+                      int ring_len, int64_t T0, const int32_t *sub_info, const int32_t *rows,
+                      int n_rows, const int32_t *tbase, int n_terms, int n_dm,
+                      const int32_t *valid_from, const double *n_prev, const double *blk_count,
+                      double *mean, double *m2, int64_t c_lo, int n_centres, int n_widths,
+                      int tile, int r_total, int q_total, float threshold, double min_count,
+                      void *cand, int cand_cap, int32_t *n_cand, dsx_stream_t stream);
 
 /* Shift-and-add combine of n_streams float32 streams (n_streams, t_stream,
  * n_inner): out[t][k] = sum_g streams[g][t0 + t - shift[g]][k] for

@@ -44,7 +44,7 @@ def test_version_and_error_paths():
     p = ctypes.cast(buf, ctypes.c_void_p)
     rc = L.dsx_corr_i8(p, None, p, 4, 2, 1, 3, 1, 1, 0, 1, None)     # odd t_int
     assert rc == -1 and b"t_int" in L.dsx_last_error()
-    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, 1, p, p, p, p, p, p, 0, 65, 1, 64, 10,
+    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, p, p, 0, 65, 1, 64, 10, 0,
                              ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
     assert rc == -1 and b"n_centres" in L.dsx_last_error()
     rc = L.dsx_image_subbands(p, p, p, p, p, 1, 1, 1, 1, 100, 1, 2, ctypes.c_float(1.0), 1, None)

@@ -1,0 +1,92 @@
+"""The dedispersion LDS plan (``dsx_dedisp_plan``, host code) checked on the
+CPU: a numpy model of the kernel's staging and summation, driven only by the
+plan's tables, must reproduce direct shift-and-add dedispersion for every DM
+and every step a tile reads, with every LDS index inside its row."""
+import numpy as np
+import pytest
+import torch
+
+from dsx import config as dcfg
+from dsx.dedisp import plan_dedisp
+from oracle import search as osearch
+
+
+def _model(plan, sh_sd, n_widths, tt, rng):
+    """Stage random subband windows, build pair rows, sum the terms."""
+    n_sub, n_dm = sh_sd.shape
+    h2 = (1 << (n_widths - 1)) // 2 if n_widths > 1 else 0
+    ww = 2 * h2
+    R = np.zeros(plan.r_total)
+    shmax = sh_sd.max(axis=1)
+    # absolute step tau of subband s sits at R[off_s + tau - (c0 - h2 - shmax_s)], c0 = 0
+    T0 = -h2 - int(shmax.max()) - 4
+    series = rng.standard_normal((n_sub, tt + ww + int(shmax.max()) - int(min(sh_sd.min(), 0)) + 8))  # tau at tau - T0
+    for s in range(n_sub):
+        off, smax, ln = plan.sub_info[s]
+        assert smax == shmax[s] and ln == tt + ww + shmax[s] - sh_sd[s].min()
+        tau = np.arange(ln) - h2 - smax
+        R[off:off + ln] = series[s, tau - T0]
+    if plan.paired:
+        X = np.zeros(plan.q_total)
+        for qoff, ln, a0, b0, blo, bhi in plan.rows:
+            assert 0 <= a0 and a0 + ln <= plan.r_total and qoff + ln <= plan.q_total
+            e = np.arange(ln)
+            v = R[a0 + e].copy()
+            ok = (b0 + e >= blo) & (b0 + e < bhi)
+            v[ok] += R[(b0 + e)[ok]]
+            X[qoff:qoff + ln] = v
+    else:
+        X = R
+    t_rel = np.arange(-h2, tt + h2)
+    D = np.zeros((n_dm, t_rel.size))
+    for g in range(plan.n_terms):
+        idx = plan.tbase[g][:, None] + t_rel[None, :]
+        assert idx.min() >= 0 and idx.max() < X.size
+        D += X[idx]
+    want = np.zeros_like(D)
+    for s in range(n_sub):
+        want += series[s, (t_rel[None, :] - sh_sd[s][:, None]) - T0]
+    return D, want
+
+
+@pytest.mark.parametrize("name,mode,dtype", [("c2", 0, torch.float16), ("c2", 2, torch.float16),
+                                              ("c2", 1, torch.float32), ("c1", 0, torch.float16)])
+def test_plan_reproduces_direct_dedispersion(name, mode, dtype):
+    cfg = dcfg.c2_config() if name == "c2" else dcfg.c1_config()
+    sh_sd = np.ascontiguousarray(cfg.shift_table().T)
+    nw = len(cfg.widths)
+    plan = plan_dedisp(sh_sd, n_widths=nw, n_centres=cfg.n_steps, ring_dtype=dtype,
+                       lds_budget=80 * 1024 if name == "c2" else 160 * 1024, mode=mode)
+    assert plan.lds_bytes <= (80 if name == "c2" else 160) * 1024
+    assert cfg.n_steps % plan.tile == 0 and plan.tile % 64 == 0
+    if mode == 1:
+        assert plan.paired
+    if mode == 2:
+        assert not plan.paired and plan.n_terms == cfg.n_sub
+    D, want = _model(plan, sh_sd, nw, plan.tile, np.random.default_rng(3))
+    np.testing.assert_allclose(D, want, rtol=1e-12, atol=1e-12)
+
+
+def test_plan_c2_is_paired_with_12_deltas():
+    cfg = dcfg.c2_config()
+    plan = plan_dedisp(cfg.shift_table().T, n_widths=6, n_centres=2048, ring_dtype=torch.float16,
+                       lds_budget=80 * 1024)
+    assert plan.paired and plan.n_terms == 8 and plan.n_rows == 96 and plan.tile == 128
+
+
+def test_plan_odd_subbands_and_signed_shifts():
+    rng = np.random.default_rng(11)
+    sh = np.sort(rng.integers(-20, 40, size=(5, 37)), axis=1).astype(np.int32)
+    for mode in (1, 2):
+        plan = plan_dedisp(sh, n_widths=4, n_centres=256, ring_dtype=torch.float32, lds_budget=160 * 1024,
+                           mode=mode)
+        D, want = _model(plan, sh, 4, plan.tile, rng)
+        np.testing.assert_allclose(D, want, rtol=1e-12, atol=1e-12)
+
+
+def test_plan_errors():
+    sh = np.zeros((4, 8), np.int32)
+    with pytest.raises(RuntimeError, match="multiple of 64"):
+        plan_dedisp(sh, n_widths=3, n_centres=100, ring_dtype=torch.float16, lds_budget=65536)
+    with pytest.raises(RuntimeError, match="LDS"):
+        plan_dedisp(sh, n_widths=3, n_centres=64, ring_dtype=torch.float16, lds_budget=100)
