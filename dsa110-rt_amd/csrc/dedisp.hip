@@ -65,9 +65,12 @@ struct DedispParams {
     double min_count;
 };
 
-constexpr int DD_CHUNK = 64;
+constexpr int DD_CHUNK = 64;      // centres per candidate chunk
+constexpr int DD_SUB = 32;        // centres per register pass (two per chunk)
 constexpr int DD_THREADS = 256;   // DMs per work-group (grid.y covers the rest)
 constexpr int DD_MAX_ROWS = 4096;
+constexpr int DD_STAGE_ITERS = 12;   // fast path: a subband window spans at most 12 x 64 steps
+constexpr int DD_STAGE_SUBS = 4;     // fast path: subbands a wave stages per batch of loads
 
 // D(t) of DM d from the term rows, in the main path's order;
 // t_rel = t - (first centre of the tile).
@@ -79,37 +82,51 @@ __device__ __forceinline__ float terms_d(const E *X, const int32_t *tbase, int n
     return acc;
 }
 
-// Pw[off + j] (=|+=) sum over terms of X[base_g + j], j < n, term bases
-// prefetched one term ahead from global memory.
-template <int N, int OFF, typename E, int NP>
-__device__ __forceinline__ void sum_terms(float (&Pw)[NP], const E *X, const int32_t *tbase, int n_terms,
-                                          int n_dm, int d, int base0, int rel) {
-    int cur = base0;
-    for (int g = 0; g < n_terms; ++g) {
-        const int nxt = g + 1 < n_terms ? tbase[(g + 1) * n_dm + d] : 0;
-        const E *xg = X + cur + rel;
-        if (g == 0) {
+// Pw[OFF + j] (=|+=) sum over terms g of X[base_g + rel + j], j < N.  With
+// MAXT > 0 the term bases sit in this lane's LDS slots (tb[h * stride], two
+// 16-bit bases per word) and the term loop is unrolled; otherwise they are
+// read from global memory term by term.
+template <int MAXT, int N, int OFF, typename E, int NP>
+__device__ __forceinline__ void sum_terms(float (&Pw)[NP], const E *X, const uint32_t *tb, int stride,
+                                          const int32_t *tbase, int n_terms, int n_dm, int d, int rel) {
+    if constexpr (MAXT > 0) {
+        {
+            const E *x0 = X + (int)(tb[0] & 0xFFFF) + rel;
 #pragma unroll
-            for (int j = 0; j < N; ++j) Pw[OFF + j] = (float)xg[j];
-        } else {
+            for (int j = 0; j < N; ++j) Pw[OFF + j] = (float)x0[j];
+        }
+#pragma unroll
+        for (int g = 1; g < MAXT; ++g) {
+            if (g < n_terms) {
+                const E *xg = X + (int)((tb[(g >> 1) * stride] >> (16 * (g & 1))) & 0xFFFF) + rel;
+#pragma unroll
+                for (int j = 0; j < N; ++j) Pw[OFF + j] += (float)xg[j];
+            }
+        }
+    } else {
+        {
+            const E *x0 = X + tbase[d] + rel;
+#pragma unroll
+            for (int j = 0; j < N; ++j) Pw[OFF + j] = (float)x0[j];
+        }
+        for (int g = 1; g < n_terms; ++g) {
+            const E *xg = X + tbase[g * n_dm + d] + rel;
 #pragma unroll
             for (int j = 0; j < N; ++j) Pw[OFF + j] += (float)xg[j];
         }
-        cur = nxt;
     }
 }
 
-template <typename T, int NW, bool PAIRED>
-__global__ __launch_bounds__(DD_THREADS) void dedisp_kernel(DedispParams P) {
+template <typename T, int NW, bool PAIRED, int MAXT>
+__global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
     constexpr int WMAX = 1 << (NW - 1);
     constexpr int H2 = WMAX / 2;
     constexpr int WW = 2 * H2;
-    constexpr int NP = DD_CHUNK + WW + 1;            // register prefix window
+    constexpr int NP = DD_SUB + WW + 1;              // register prefix window
     extern __shared__ float lds[];
     float *Q = lds;                                   // [q_total] float32 pair rows
-    T *R = reinterpret_cast<T *>(lds + P.q_total);    // [r_total] subband windows
-    using E = typename std::conditional<PAIRED, float, T>::type;
-    const E *X = PAIRED ? reinterpret_cast<const E *>(Q) : reinterpret_cast<const E *>(R);
+    float *R = lds + P.q_total;                       // [r_total] subband windows
+    const float *X = PAIRED ? Q : R;                  // rows the lanes sum
     const T *ring = reinterpret_cast<const T *>(P.ring);
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
@@ -117,27 +134,51 @@ __global__ __launch_bounds__(DD_THREADS) void dedisp_kernel(DedispParams P) {
     const bool active = d < P.n_dm;
     const int64_t c_hi = P.c_lo + P.n_centres;
 
-    const int tb0 = active ? P.tbase[d] : 0;
+    // term bases of this lane, two 16-bit LDS indices per word, in LDS slots
+    // after the windows (kept out of registers: the chunk loop needs them all)
+    constexpr int TBW = MAXT > 0 ? (MAXT + 1) / 2 : 1;
+    uint32_t *tb = reinterpret_cast<uint32_t *>(lds + P.q_total + P.r_total) + tid;
+    // subband window table (offset, shmax, length) copied to LDS, so the staging
+    // loop's table reads never wait behind the window loads in flight
+    int *si = reinterpret_cast<int *>(lds + P.q_total + P.r_total) + (MAXT > 0 ? TBW * nthr : 0);
+    for (int i = tid; i < 3 * P.n_sub; i += nthr) si[i] = P.sub_info[i];
+    if constexpr (MAXT > 0) {
+#pragma unroll
+        for (int h = 0; h < TBW; ++h) {
+            uint32_t lo = 0, hi = 0;
+            if (active && 2 * h < P.n_terms) lo = (uint32_t)P.tbase[(2 * h) * P.n_dm + d];
+            if (active && 2 * h + 1 < P.n_terms) hi = (uint32_t)P.tbase[(2 * h + 1) * P.n_dm + d];
+            tb[h * nthr] = lo | (hi << 16);
+        }
+    }
+    const int lane = tid & 63;
+    const int n_waves = nthr >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t plane = (int64_t)P.n_pix * P.row_len;
+    __syncthreads();                                      // si[] visible
+    // fast staging: a wave owns at most DD_STAGE_SUBS subbands (s = wave +
+    // u * n_waves) whose windows span at most DD_STAGE_ITERS x 64 steps
+    bool fast = MAXT > 0 && P.n_sub <= n_waves * DD_STAGE_SUBS;
+    for (int s = 0; s < P.n_sub; ++s) fast = fast && si[3 * s + 2] <= 64 * DD_STAGE_ITERS;
 
     for (int p = blockIdx.x; p < P.n_pix; p += gridDim.x) {
         // ---- per-lane state from earlier blocks (float64)
-        float wmu[NW], inv[NW];
+        // w_k mean and 1 / (sigma sqrt(w_k)) are power-of-two scalings of the
+        // k = 0 / k = 1 values (exact in binary floating point), so three floats
+        // carry them: wmu_k = 2^k wmu0, inv_k = 2^-(k/2) inv_{k mod 2}
+        float wmu0 = 0.f, inv0 = 0.f, inv1 = 0.f;
         bool score = false;
         int vf = 0;
-        double mean_a = 0.0, m2_a = 0.0, n_a = 0.0;
         if (active) {
             vf = P.valid_from[d];
-            n_a = P.n_prev[d];
-            mean_a = P.mean[(int64_t)p * P.n_dm + d];
-            m2_a = P.m2[(int64_t)p * P.n_dm + d];
+            const double n_a = P.n_prev[d];
+            const double mean_a = P.mean[(int64_t)p * P.n_dm + d];
+            const double m2_a = P.m2[(int64_t)p * P.n_dm + d];
             const double sig = n_a > 0.0 ? sqrt(m2_a / n_a) : 0.0;
             score = (n_a >= P.min_count) && (n_a > 0.0) && (sig > 0.0);
-#pragma unroll
-            for (int k = 0; k < NW; ++k) {
-                const double w = (double)(1 << k);
-                wmu[k] = (float)(w * mean_a);
-                inv[k] = score ? (float)(1.0 / (sig * sqrt(w))) : 0.f;
-            }
+            wmu0 = (float)mean_a;
+            inv0 = score ? (float)(1.0 / sig) : 0.f;
+            inv1 = score ? (float)(1.0 / (sig * sqrt(2.0))) : 0.f;
         }
         double s1 = 0.0, s2 = 0.0;
         float m1_tail = -INFINITY;    // max of D over [c, c + H2) for the next chunk
@@ -147,33 +188,82 @@ __global__ __launch_bounds__(DD_THREADS) void dedisp_kernel(DedispParams P) {
 
         for (int c0 = 0; c0 < P.n_centres; c0 += P.tt) {
             const int64_t cabs0 = P.c_lo + c0;
-            // ---- stage every subband's window (ring dtype)
+            // ---- stage every subband's window (ring dtype): step tau of subband s
+            //      lands at R[off_s + tau - (cabs0 - H2 - shmax_s)]
             __syncthreads();
-            for (int s = 0; s < P.n_sub; ++s) {
-                const int off = P.sub_info[3 * s];
-                const int shmax = P.sub_info[3 * s + 1];
-                const int len = P.sub_info[3 * s + 2];
-                const int64_t tau_lo = cabs0 - H2 - shmax;
-                const T *row = ring + ((int64_t)s * P.n_pix + p) * P.row_len;
-                for (int e = tid; e < len; e += nthr) {
-                    const int64_t pos = P.hist + (tau_lo + e) - P.T0;
-                    T v = (T)0;
-                    if (pos >= 0 && pos < P.row_len) v = row[pos];
-                    R[off + e] = v;
+            const int64_t pos0 = (int64_t)P.hist + cabs0 - H2 - P.T0;   // ring position of e - shmax_s = 0
+            const T *ring_p = ring + (int64_t)p * P.row_len;
+            if (MAXT > 0 && fast) {
+                // each wave issues the loads of all its subbands before its first
+                // store (unconditional loads: clamped index, zero selected
+                // afterwards), so they are in flight together
+                float v[MAXT > 0 ? DD_STAGE_SUBS : 1][MAXT > 0 ? DD_STAGE_ITERS : 1];
+#pragma unroll
+                for (int u = 0; u < (MAXT > 0 ? DD_STAGE_SUBS : 1); ++u) {
+                    const int s = wave + u * n_waves;
+                    const bool live = s < P.n_sub;
+                    const int len = live ? si[3 * s + 2] : 0;
+                    const T *row = ring_p + (int64_t)(live ? s : 0) * plane;
+                    const int64_t q0 = pos0 - (live ? si[3 * s + 1] : 0);
+#pragma unroll
+                    for (int k = 0; k < (MAXT > 0 ? DD_STAGE_ITERS : 1); ++k) {
+                        const int e = lane + 64 * k;
+                        const int64_t pos = q0 + e;
+                        const bool ok = e < len && pos >= 0 && pos < P.row_len;
+                        const T x = row[ok ? pos : 0];
+                        v[u][k] = ok ? (float)x : 0.f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < (MAXT > 0 ? DD_STAGE_SUBS : 1); ++u) {
+                    const int s = wave + u * n_waves;
+                    if (s < P.n_sub) {
+                        const int off = si[3 * s];
+                        const int len = si[3 * s + 2];
+#pragma unroll
+                        for (int k = 0; k < (MAXT > 0 ? DD_STAGE_ITERS : 1); ++k)
+                            if (lane + 64 * k < len) R[off + lane + 64 * k] = v[u][k];
+                    }
+                }
+            } else {
+                // generic staging: each wave stages whole subbands (wave-uniform s),
+                // four loads in flight per lane
+                for (int s = wave; s < P.n_sub; s += n_waves) {
+                    const int off = si[3 * s];
+                    const int len = si[3 * s + 2];
+                    const T *row = ring_p + (int64_t)s * plane;
+                    const int64_t q0 = pos0 - si[3 * s + 1];      // ring position of e = 0
+                    for (int e0 = lane; e0 < len; e0 += 4 * 64) {
+                        float v[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const int e = e0 + 64 * k;
+                            const int64_t pos = q0 + e;
+                            const bool ok = e < len && pos >= 0 && pos < P.row_len;
+                            const T x = row[ok ? pos : 0];
+                            v[k] = ok ? (float)x : 0.f;
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if (e0 + 64 * k < len) R[off + e0 + 64 * k] = v[k];
+                    }
                 }
             }
             __syncthreads();
-            // ---- one float32 row per (pair, delta): Q[e] = R[a0 + e] + R[b0 + e]
+            // ---- one float32 row per (pair, delta): Q[e] = R[a0 + e] + R[b0 + e];
+            //      each wave builds every n_waves-th row
             if (PAIRED) {
-                for (int r = 0; r < P.n_rows; ++r) {
+                for (int r = wave; r < P.n_rows; r += n_waves) {
                     const int qoff = P.rows[6 * r], len = P.rows[6 * r + 1];
                     const int a0 = P.rows[6 * r + 2], b0 = P.rows[6 * r + 3];
                     const int blo = P.rows[6 * r + 4], bhi = P.rows[6 * r + 5];
-                    for (int e = tid; e < len; e += nthr) {
-                        float v = (float)R[a0 + e];
+#pragma unroll 4
+                    for (int e = lane; e < len; e += 64) {
                         const int ib = b0 + e;
-                        if (ib >= blo && ib < bhi) v += (float)R[ib];
-                        Q[qoff + e] = v;
+                        const bool okb = ib >= blo && ib < bhi;
+                        const float va = R[a0 + e];
+                        const float vb = R[okb ? ib : a0 + e];
+                        Q[qoff + e] = okb ? va + vb : va;
                     }
                 }
             }
@@ -182,7 +272,7 @@ __global__ __launch_bounds__(DD_THREADS) void dedisp_kernel(DedispParams P) {
 
             if (c0 == 0) {
                 // prologue: D(c_lo - H2 + x), x < WW, into Pw[x + 1]
-                if (WW > 0) sum_terms<WW, 1>(Pw, X, P.tbase, P.n_terms, P.n_dm, d, tb0, -H2);
+                if (WW > 0) sum_terms<MAXT, WW, 1>(Pw, X, tb, nthr, P.tbase, P.n_terms, P.n_dm, d, -H2);
                 // samples t = c_lo + (x - H2) for x >= H2 belong to this block
                 float a1 = 0.f, a2 = 0.f, m = -INFINITY;
 #pragma unroll
@@ -203,89 +293,109 @@ __global__ __launch_bounds__(DD_THREADS) void dedisp_kernel(DedispParams P) {
             }
 
             for (int cc = 0; cc < P.tt; cc += DD_CHUNK) {
-                const int64_t c = cabs0 + cc;
-                // ---- D(c + H2 + i), i < 64, summed over the term rows into the window slots
-                sum_terms<DD_CHUNK, WW + 1>(Pw, X, P.tbase, P.n_terms, P.n_dm, d, tb0, cc + H2);
-                // ---- statistics of this block's complete samples, width-1 maxima
-                const int64_t t1 = c + H2;
-                float a1 = 0.f, a2 = 0.f;
-                const bool full = t1 >= vf && t1 + DD_CHUNK <= c_hi;
-                if (full) {
+                const int64_t c = cabs0 + cc;                  // first centre of the 64-centre chunk
+                const float pbase = Pw[0];                       // P[c - H2], for the rescan
+                const bool scored = score && c - H2 >= vf;
+                float mb[NW];
 #pragma unroll
-                    for (int i = 0; i < DD_CHUNK; ++i) a2 = fmaf(Pw[WW + 1 + i], Pw[WW + 1 + i], a2);
-                } else {
+                for (int k = 0; k < NW; ++k) mb[k] = -INFINITY;
+                float m1n = -INFINITY;                           // width-1 maximum of the next chunk
+                mb[0] = m1_tail;
 #pragma unroll
-                    for (int i = 0; i < DD_CHUNK; ++i) {
-                        const float v = Pw[WW + 1 + i];
-                        const bool ok = t1 + i >= vf && t1 + i < c_hi;
-                        a1 += ok ? v : 0.f;
-                        a2 = fmaf(ok ? v : 0.f, v, a2);
+                for (int j = 0; j < DD_CHUNK / DD_SUB; ++j) {
+                    const int cs = cc + j * DD_SUB;                  // sub-chunk start (tile relative)
+                    // ---- D(c + j S + H2 + i), i < S, summed over the term rows
+                    sum_terms<MAXT, DD_SUB, WW + 1>(Pw, X, tb, nthr, P.tbase, P.n_terms, P.n_dm, d, cs + H2);
+                    // ---- statistics of this block's complete samples
+                    const int64_t t1 = c + j * DD_SUB + H2;
+                    float a1 = 0.f, a2 = 0.f;
+                    const bool full = t1 >= vf && t1 + DD_SUB <= c_hi;
+                    if (full) {
+#pragma unroll
+                        for (int i = 0; i < DD_SUB; ++i) a2 = fmaf(Pw[WW + 1 + i], Pw[WW + 1 + i], a2);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < DD_SUB; ++i) {
+                            const float v = Pw[WW + 1 + i];
+                            const bool ok = t1 + i >= vf && t1 + i < c_hi;
+                            a1 += ok ? v : 0.f;
+                            a2 = fmaf(ok ? v : 0.f, v, a2);
+                        }
                     }
+                    // ---- width-1 maxima from the fresh samples: t < c + 64 belong to this chunk
+                    const int split = DD_CHUNK - H2 - j * DD_SUB;
+#pragma unroll
+                    for (int i = 0; i < DD_SUB; ++i) {
+                        if (i < split) mb[0] = fmaxf(mb[0], Pw[WW + 1 + i]);
+                        else m1n = fmaxf(m1n, Pw[WW + 1 + i]);
+                    }
+                    // ---- prefix sums in place
+#pragma unroll
+                    for (int i = 0; i < DD_SUB; ++i) Pw[WW + 1 + i] += Pw[WW + i];
+                    if (full) a1 = Pw[WW + DD_SUB] - Pw[WW];
+                    s1 += (double)a1;
+                    s2 += (double)a2;
+                    // ---- boxcar maxima of the wider widths over this sub-chunk's centres
+                    if (scored) {
+#pragma unroll
+                        for (int k = 1; k < NW; ++k) {
+                            const int w = 1 << k;
+                            const int lo = H2 - (w >> 1);
+                            float m = mb[k];
+#pragma unroll
+                            for (int i = 0; i < DD_SUB; ++i) m = fmaxf(m, Pw[i + lo + w] - Pw[i + lo]);
+                            mb[k] = m;
+                        }
+                    }
+                    // ---- slide the prefix window by one sub-chunk
+#pragma unroll
+                    for (int x = 0; x <= WW; ++x) Pw[x] = Pw[x + DD_SUB];
                 }
-                float mA = -INFINITY, mB = -INFINITY;
-#pragma unroll
-                for (int i = 0; i < DD_CHUNK - H2; ++i) mA = fmaxf(mA, Pw[WW + 1 + i]);
-#pragma unroll
-                for (int i = DD_CHUNK - H2; i < DD_CHUNK; ++i) mB = fmaxf(mB, Pw[WW + 1 + i]);
-                const float m1 = fmaxf(m1_tail, mA);
-                m1_tail = mB;
-                // ---- prefix sums in place
-#pragma unroll
-                for (int i = 0; i < DD_CHUNK; ++i) Pw[WW + 1 + i] += Pw[WW + i];
-                if (full) a1 = Pw[WW + DD_CHUNK] - Pw[WW];
-                s1 += (double)a1;
-                s2 += (double)a2;
+                m1_tail = m1n;
 
-                // ---- boxcar chunk maxima and candidate
-                if (score && c - H2 >= vf) {
-                    float mb[NW];
-                    mb[0] = m1;
-#pragma unroll
-                    for (int k = 1; k < NW; ++k) {
-                        const int w = 1 << k;
-                        const int lo = H2 - (w >> 1);
-                        float m = -INFINITY;
-#pragma unroll
-                        for (int i = 0; i < DD_CHUNK; ++i) m = fmaxf(m, Pw[i + lo + w] - Pw[i + lo]);
-                        mb[k] = m;
-                    }
+                // ---- chunk maximum over widths and candidate
+                if (scored) {
                     float best = -INFINITY;
                     int bk = 0;
 #pragma unroll
                     for (int k = 0; k < NW; ++k) {
-                        const float snr = (mb[k] - wmu[k]) * inv[k];
+                        const float snr = (mb[k] - ldexpf(wmu0, k)) * ldexpf((k & 1) ? inv1 : inv0, -(k >> 1));
                         if (snr > best) {
                             best = snr;
                             bk = k;
                         }
                     }
                     if (best > P.threshold) {
-                        // rare path: earliest centre of the winning width, from
-                        // values computed exactly as the maxima were
+                        // rare path: earliest centre of the winning width.  D and the
+                        // prefix chain are recomputed from the LDS rows in the main
+                        // path's order, so the boxcars are bit-identical to the maxima.
+                        const float target = mb[bk];
                         int bi = 0;
                         if (bk == 0) {
                             for (int i = 0; i < DD_CHUNK; ++i) {
-                                if (terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc + i) == mb[0]) {
+                                if (terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc + i) == target) {
                                     bi = i;
                                     break;
                                 }
                             }
                         } else {
-#pragma unroll
-                            for (int k = 1; k < NW; ++k) {
-                                if (k == bk) {
-                                    const int w = 1 << k;
-                                    const int lo = H2 - (w >> 1);
-#pragma unroll
-                                    for (int i = DD_CHUNK - 1; i >= 0; --i) {
-                                        // the empty asm keeps the compiler from reusing the
-                                        // maxima phase's differences (which would keep all
-                                        // of them live until here); the values are the same
-                                        float vh = Pw[i + lo + w], vl = Pw[i + lo];
-                                        asm volatile("" : "+v"(vh), "+v"(vl));
-                                        if (vh - vl == mb[k]) bi = i;
-                                    }
+                            const int w = 1 << bk;
+                            const int lo = H2 - (w >> 1);
+                            // chains at prefix positions x = i + lo and x = i + lo + w, where
+                            // position x is P[c - H2 + x] = pbase + sum_{x' < x} D(c - H2 + x')
+                            float pa = pbase, pb = pbase;
+                            for (int x = 0; x < lo + w; ++x) {
+                                const float dv = terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc - H2 + x);
+                                if (x < lo) pa = dv + pa;
+                                pb = dv + pb;
+                            }
+                            for (int i = 0; i < DD_CHUNK; ++i) {
+                                if (pb - pa == target) {
+                                    bi = i;
+                                    break;
                                 }
+                                pa = terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc - H2 + lo + i) + pa;
+                                pb = terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc - H2 + lo + w + i) + pb;
                             }
                         }
                         const int slot = atomicAdd(P.n_cand, 1);
@@ -300,15 +410,16 @@ __global__ __launch_bounds__(DD_THREADS) void dedisp_kernel(DedispParams P) {
                         }
                     }
                 }
-                // ---- slide the prefix window by one chunk
-#pragma unroll
-                for (int x = 0; x <= WW; ++x) Pw[x] = Pw[x + DD_CHUNK];
             }
         }
         // ---- merge this block's statistics into the running state
         if (active) {
             const double nb = P.blk_count[d];
             if (nb > 0.0) {
+                // earlier blocks' state, read again rather than kept in registers
+                const double n_a = P.n_prev[d];
+                const double mean_a = P.mean[(int64_t)p * P.n_dm + d];
+                const double m2_a = P.m2[(int64_t)p * P.n_dm + d];
                 const double mean_b = s1 / nb;
                 const double m2_b = s2 - s1 * s1 / nb;
                 const double n = n_a + nb;
@@ -320,12 +431,20 @@ __global__ __launch_bounds__(DD_THREADS) void dedisp_kernel(DedispParams P) {
     }
 }
 
+// Plans with at most 8 (or 16) terms keep the lanes' term bases in registers.
+template <typename T, int NW, bool PAIRED, int MAXT>
+static int launch_maxt(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
+    (void)hipFuncSetAttribute((const void *)dedisp_kernel<T, NW, PAIRED, MAXT>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    dedisp_kernel<T, NW, PAIRED, MAXT><<<grid, threads, lds, s>>>(P);
+    return check_launch("dsx_dedisp_detect");
+}
+
 template <typename T, int NW, bool PAIRED>
 static int launch_dedisp(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
-    (void)hipFuncSetAttribute((const void *)dedisp_kernel<T, NW, PAIRED>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    dedisp_kernel<T, NW, PAIRED><<<grid, threads, lds, s>>>(P);
-    return check_launch("dsx_dedisp_detect");
+    if (P.n_terms <= 8) return launch_maxt<T, NW, PAIRED, 8>(P, threads, lds, grid, s);
+    if (P.n_terms <= 16) return launch_maxt<T, NW, PAIRED, 16>(P, threads, lds, grid, s);
+    return launch_maxt<T, NW, PAIRED, 0>(P, threads, lds, grid, s);
 }
 
 template <typename T, bool PAIRED>
@@ -341,6 +460,9 @@ static int dispatch_nw(int nw, const DedispParams &P, int threads, size_t lds, d
         default: return set_error(DSX_ENOTSUP, "dsx_dedisp_detect: n_widths=%d not in [1, 7]", nw);
     }
 }
+
+// LDS words per thread holding its term bases (matches the kernel's MAXT choice).
+static int tb_words(int n_terms) { return n_terms <= 8 ? 4 : n_terms <= 16 ? 8 : 0; }
 
 // Pair rows of a plan: per pair g the sorted distinct deltas and, per delta,
 // the range [lo, hi] of sh_a over the DMs that use it.
@@ -395,7 +517,6 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
                 "dsx_dedisp_plan: n_centres must be a positive multiple of 64");
     DSX_REQUIRE(ring_dtype == DSX_F16 || ring_dtype == DSX_F32, "dsx_dedisp_plan: ring_dtype");
     DSX_REQUIRE(mode >= 0 && mode <= 2, "dsx_dedisp_plan: mode=%d not in {0, 1, 2}", mode);
-    const int64_t esize = ring_dtype == DSX_F16 ? 2 : 4;
     const int ww = n_widths > 1 ? (1 << (n_widths - 1)) : 0;
     const int h2 = ww / 2;
     std::vector<int> shmax(n_sub), shmin(n_sub);
@@ -420,13 +541,22 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
         for (int r = 0; r < n_prow; ++r) q += tt + ww + pr.hi[r] - pr.lo[r];
         return q;
     };
+    // largest tile within the LDS budget, preferring tiles whose windows fit
+    // the batched staging of the fast kernel
+    const int threads = std::min(DD_THREADS, ((n_dm + 63) / 64) * 64);
     auto best_tile = [&](bool paired) {
-        int best = 0;
+        int best = 0, best_fast = 0;
+        const int n_terms = paired ? (n_sub + 1) / 2 : n_sub;
         for (int t = DD_CHUNK; t <= n_centres && t <= 2048; t += DD_CHUNK) {
-            const int64_t bytes = (paired ? q_elems(t) * 4 : 0) + r_elems(t) * esize;
-            if (n_centres % t == 0 && bytes <= lds_budget) best = t;
+            const int64_t bytes = (paired ? q_elems(t) * 4 : 0) + r_elems(t) * 4 +
+                                  ((int64_t)tb_words(n_terms) * threads + 3 * n_sub) * 4;
+            if (n_centres % t != 0 || bytes > lds_budget) continue;
+            best = t;
+            bool fits = true;
+            for (int s2 = 0; s2 < n_sub; ++s2) fits = fits && t + ww + shmax[s2] - shmin[s2] <= 64 * DD_STAGE_ITERS;
+            if (fits) best_fast = t;
         }
-        return best;
+        return best_fast > 0 ? best_fast : best;
     };
     bool paired = false;
     int tt = 0;
@@ -437,12 +567,12 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
     if (!paired) {
         if (mode == 1)
             return set_error(DSX_ERANGE, "dsx_dedisp_plan: pair rows need %lld B of LDS > %d",
-                             (long long)(q_elems(DD_CHUNK) * 4 + r_elems(DD_CHUNK) * esize), lds_budget);
+                             (long long)(q_elems(DD_CHUNK) * 4 + r_elems(DD_CHUNK) * 4), lds_budget);
         tt = best_tile(false);
     }
     if (tt == 0)
         return set_error(DSX_ERANGE, "dsx_dedisp_plan: the 64-centre tile needs %lld B of LDS > %d",
-                         (long long)(r_elems(DD_CHUNK) * esize), lds_budget);
+                         (long long)(r_elems(DD_CHUNK) * 4), lds_budget);
     const int n_rows = paired ? n_prow : 0;
     if (n_rows > rows_cap)
         return set_error(DSX_ERANGE, "dsx_dedisp_plan: %d pair rows > rows_cap %d", n_rows, rows_cap);
@@ -495,12 +625,14 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
             for (int d = 0; d < n_dm; ++d)
                 tbase[(int64_t)s * n_dm + d] = roff[s] + h2 + shmax[s] - host_shift[(int64_t)s * n_dm + d];
     }
+    const int n_terms = paired ? (n_sub + 1) / 2 : n_sub;
     info[0] = tt;
     info[1] = n_rows;
-    info[2] = paired ? (n_sub + 1) / 2 : n_sub;   // terms per dedispersed sample
+    info[2] = n_terms;                              // terms per dedispersed sample
     info[3] = r_total;
     info[4] = q_total;
     info[5] = paired ? 1 : 0;
+    info[6] = (int)(((int64_t)q_total + r_total + (int64_t)tb_words(n_terms) * threads + 3 * n_sub) * 4);
     return DSX_OK;
 }
 
@@ -530,8 +662,9 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
                 "dsx_dedisp_detect: tile must be a multiple of 64 dividing n_centres");
     DSX_REQUIRE(n_widths >= 1 && n_widths <= 7, "dsx_dedisp_detect: n_widths=%d", n_widths);
     DSX_REQUIRE(cand_cap >= 0 && r_total > 0, "dsx_dedisp_detect: sizes");
-    const size_t esize = ring_dtype == DSX_F16 ? 2 : 4;
-    const size_t lds = (size_t)q_total * 4 + (size_t)r_total * esize;
+    const int threads = std::min(DD_THREADS, ((n_dm + 63) / 64) * 64);
+    const size_t lds =
+        ((size_t)q_total + (size_t)r_total + (size_t)tb_words(n_terms) * threads + 3 * (size_t)n_sub) * 4;
     if (lds > 160 * 1024)
         return set_error(DSX_ERANGE, "dsx_dedisp_detect: %zu B of LDS for the time windows", lds);
     DedispParams P{};
@@ -562,7 +695,6 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
     P.cand_cap = cand_cap;
     P.threshold = threshold;
     P.min_count = min_count;
-    const int threads = std::min(DD_THREADS, ((n_dm + 63) / 64) * 64);
     const dim3 grid((unsigned)std::min(n_pix, 1 << 20), (unsigned)((n_dm + DD_THREADS - 1) / DD_THREADS));
     hipStream_t s = as_stream(stream);
     if (ring_dtype == DSX_F16)

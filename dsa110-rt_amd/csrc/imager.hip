@@ -23,6 +23,51 @@ constexpr int TAP_CONJ = 1 << 20;
 constexpr int TAP_NEG = 1 << 21;
 constexpr int TAP_CELL = (1 << 20) - 1;
 
+// In-register 16-point transform X[k] = sum_n x[n] e^{+2 pi i n k / 16} (natural
+// order in and out), as 4 x 4: four 4-point transforms over n1 (n = 4 n1 + n2),
+// twiddles e^{2 pi i n2 k1 / 16}, four 4-point transforms over n2 (k = k1 + 4 k2).
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ void dft4(float2 &a, float2 &b, float2 &c, float2 &d) {
+    // X0 = a+b+c+d, X1 = a + i b - c - i d, X2 = a - b + c - d, X3 = a - i b - c + i d
+    const float2 s0 = make_float2(a.x + c.x, a.y + c.y), d0 = make_float2(a.x - c.x, a.y - c.y);
+    const float2 s1 = make_float2(b.x + d.x, b.y + d.y), d1 = make_float2(b.x - d.x, b.y - d.y);
+    a = make_float2(s0.x + s1.x, s0.y + s1.y);
+    c = make_float2(s0.x - s1.x, s0.y - s1.y);
+    b = make_float2(d0.x - d1.y, d0.y + d1.x);      // d0 + i d1
+    d = make_float2(d0.x + d1.y, d0.y - d1.x);      // d0 - i d1
+}
+
+__device__ __forceinline__ void dft16(float2 (&x)[16]) {
+    constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f, R2 = 0.70710678118654752f;
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) dft4(x[n2], x[4 + n2], x[8 + n2], x[12 + n2]);   // -> A[n2][k1] at x[4 k1 + n2]
+    // twiddles e^{2 pi i n2 k1 / 16} for n2, k1 in 1..3
+    const float2 w1 = make_float2(C1, S1), w2 = make_float2(R2, R2), w3 = make_float2(S1, C1);
+    const float2 w6 = make_float2(-R2, R2), w9 = make_float2(-C1, -S1);
+    x[4 + 1] = cmul(x[4 + 1], w1);
+    x[4 + 2] = cmul(x[4 + 2], w2);
+    x[4 + 3] = cmul(x[4 + 3], w3);
+    x[8 + 1] = cmul(x[8 + 1], w2);
+    x[8 + 2] = make_float2(-x[8 + 2].y, x[8 + 2].x);    // i
+    x[8 + 3] = cmul(x[8 + 3], w6);
+    x[12 + 1] = cmul(x[12 + 1], w3);
+    x[12 + 2] = cmul(x[12 + 2], w6);
+    x[12 + 3] = cmul(x[12 + 3], w9);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) dft4(x[4 * k1], x[4 * k1 + 1], x[4 * k1 + 2], x[4 * k1 + 3]);
+    // X[k1 + 4 k2] sits at x[4 k1 + k2]: transpose the 4 x 4 index
+    float2 y[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) y[k1 + 4 * k2] = x[4 * k1 + k2];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = y[k];
+}
+
 struct ImgParams {
     const int32_t *vis;
     const int32_t *tap_src;
@@ -35,18 +80,29 @@ struct ImgParams {
     int dense;                // 0: taps from int32 visibilities (+ mask); 1: dense grid, no mask
 };
 
-template <typename TOut>
+// R16: N = 256 row transforms as two in-register 16-point passes (rows padded
+// to RS = 272 entries so the pass-1 transposed writes are conflict-free);
+// otherwise radix-2 passes over rows of RS = N entries.
+constexpr int R16_RS = 272;
+
+template <typename TOut, bool R16>
 __global__ __launch_bounds__(IMG_THREADS) void image_kernel(ImgParams P) {
     extern __shared__ __align__(16) float2 smem[];
     const int N = P.N;
+    const int RS = R16 ? R16_RS : N;         // row stride of H
     float2 *tw = smem;                       // N entries: e^{+2 pi i m / N}
-    float2 *H = smem + N;                    // rc_max x N
+    float2 *tw16 = smem + N;                 // R16: [j1][w2] = e^{+2 pi i w2 j1 / 256}
+    float2 *H = smem + N + (R16 ? 256 : 0);  // rc_max x RS
     int2 *Hi = reinterpret_cast<int2 *>(H);
     const int tid = threadIdx.x;
     for (int m = tid; m < N; m += IMG_THREADS) {
         double sv, cv;
         sincospi(2.0 * (double)m / (double)N, &sv, &cv);
         tw[m] = make_float2((float)cv, (float)sv);
+    }
+    if (R16) {
+        __syncthreads();
+        for (int m = tid; m < 256; m += IMG_THREADS) tw16[m] = tw[((m >> 4) * (m & 15)) & 255];
     }
     const int wave = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lg = lane >> 4;
@@ -79,12 +135,12 @@ __global__ __launch_bounds__(IMG_THREADS) void image_kernel(ImgParams P) {
                     const float2 a = gsrc[(int64_t)rg * N + w];
                     const float2 b = gsrc[(int64_t)((N - rg) & (N - 1)) * N + ((N - w) & (N - 1))];
                     const float sg = ((rg + w) & 1) ? -1.f : 1.f;
-                    const int wr = (int)(__brev((unsigned)w) >> (32 - P.log2N));
-                    H[r * N + wr] = make_float2(sg * (a.x + b.x), sg * (a.y - b.y));
+                    const int wr = R16 ? w : (int)(__brev((unsigned)w) >> (32 - P.log2N));
+                    H[r * RS + wr] = make_float2(sg * (a.x + b.x), sg * (a.y - b.y));
                 }
                 __syncthreads();
             } else {
-            for (int i = tid; i < rc * N; i += IMG_THREADS) Hi[i] = make_int2(0, 0);
+            for (int i = tid; i < rc * RS; i += IMG_THREADS) Hi[i] = make_int2(0, 0);
             __syncthreads();
             // ---- 1. scatter taps (int32 LDS atomics)
             for (int i = tap_lo + tid; i < tap_hi; i += IMG_THREADS) {
@@ -96,12 +152,14 @@ __global__ __launch_bounds__(IMG_THREADS) void image_kernel(ImgParams P) {
                 int2 v = *reinterpret_cast<const int2 *>(vslab + 2 * (int64_t)P.tap_src[i]);
                 if (d & TAP_CONJ) v.y = -v.y;
                 if (d & TAP_NEG) { v.x = -v.x; v.y = -v.y; }
-                int *c = reinterpret_cast<int *>(&Hi[r * N + w]);
+                int *c = reinterpret_cast<int *>(&Hi[r * RS + w]);
                 atomicAdd(c, v.x);
                 atomicAdd(c + 1, v.y);
             }
             __syncthreads();
             // ---- int32 -> float32 in place, with the bit-reversal permutation
+            //      (R16: the first 16-point pass converts as it loads)
+            if (!R16)
             for (int i = tid; i < rc * N; i += IMG_THREADS) {
                 const int r = i >> P.log2N, j = i & (N - 1);
                 const int jr = (int)(__brev((unsigned)j) >> (32 - P.log2N));
@@ -116,13 +174,50 @@ __global__ __launch_bounds__(IMG_THREADS) void image_kernel(ImgParams P) {
             }
             __syncthreads();
             }
-            // ---- 2. radix-2 DIT row transforms
-            const int halfN = N >> 1;
+            // ---- 2. row transforms Y[r][j] = sum_w H[r][w] e^{+2 pi i w j / N}
+            if constexpr (R16) {
+                // pass 1 (task = row, w2): Z[w2][j1] = e^{2 pi i w2 j1 / 256} sum_w1 H[16 w1 + w2]
+                // e^{2 pi i w1 j1 / 16}, stored at 17 w2 + j1.  The 16 tasks of a row are
+                // 16 lanes of one wave, so a row's reads finish before its writes.
+                const bool h_int = !P.dense;
+                for (int task = tid; task < rc * 16; task += IMG_THREADS) {
+                    const int r = task >> 4, w2 = task & 15;
+                    float2 x[16];
+#pragma unroll
+                    for (int w1 = 0; w1 < 16; ++w1) {
+                        const int idx = r * RS + 16 * w1 + w2;
+                        if (h_int) {
+                            const int2 v = Hi[idx];
+                            x[w1] = make_float2((float)v.x, (float)v.y);
+                        } else {
+                            x[w1] = H[idx];
+                        }
+                    }
+                    dft16(x);
+#pragma unroll
+                    for (int j1 = 1; j1 < 16; ++j1) x[j1] = cmul(x[j1], tw16[j1 * 16 + w2]);
+#pragma unroll
+                    for (int j1 = 0; j1 < 16; ++j1) H[r * RS + 17 * w2 + j1] = x[j1];
+                }
+                __syncthreads();
+                // pass 2 (task = row, j1): Y[j1 + 16 j2] = sum_w2 Z[w2][j1] e^{2 pi i w2 j2 / 16}
+                for (int task = tid; task < rc * 16; task += IMG_THREADS) {
+                    const int r = task >> 4, j1 = task & 15;
+                    float2 x[16];
+#pragma unroll
+                    for (int w2 = 0; w2 < 16; ++w2) x[w2] = H[r * RS + 17 * w2 + j1];
+                    dft16(x);
+#pragma unroll
+                    for (int j2 = 0; j2 < 16; ++j2) H[r * RS + j1 + 16 * j2] = x[j2];
+                }
+                __syncthreads();
+            } else {
+            const int halfN_ = N >> 1;
             for (int m = 1; m < N; m <<= 1) {
                 const int stride = N / (2 * m);
-                for (int b = tid; b < rc * halfN; b += IMG_THREADS) {
-                    const int r = b / halfN;
-                    const int k = b - r * halfN;
+                for (int b = tid; b < rc * halfN_; b += IMG_THREADS) {
+                    const int r = b / halfN_;
+                    const int k = b - r * halfN_;
                     const int pos = k & (m - 1);
                     const int i0 = ((k - pos) << 1) + pos;
                     const float2 w = tw[pos * stride];
@@ -134,6 +229,8 @@ __global__ __launch_bounds__(IMG_THREADS) void image_kernel(ImgParams P) {
                 }
                 __syncthreads();
             }
+            }
+            const int halfN = N >> 1;
             // ---- 3. column C2R on MFMA: D[i][j] = sum_k A[i][k] B[k][j],
             //         k = 2 r + part: A = (cos, -sin)(2 pi r i / N), B = c_r (Re, Im) Y[r][j]
             const int nkb = (2 * rc + 3) / 4;
@@ -166,7 +263,7 @@ __global__ __launch_bounds__(IMG_THREADS) void image_kernel(ImgParams P) {
                             if (r < rc) {
                                 const int rg = r0 + r;
                                 const float cr = (rg == 0 || rg == halfN) ? 1.f : 2.f;
-                                const float2 y = H[r * N + j0 + lr];
+                                const float2 y = H[r * RS + j0 + lr];
                                 bv = cr * ((k & 1) ? y.y : y.x);
                             }
                             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[kb], bv, acc, 0, 0, 0);
@@ -212,6 +309,28 @@ __global__ __launch_bounds__(256) void grid_gather_kernel(const T *vis, const in
     }
 }
 
+// uv rows per LDS pass: the twiddle tables plus rc rows of RS float2 in 96 KB
+static int img_rows_per_chunk(int n_grid, bool r16) {
+    const size_t budget = 96 * 1024;
+    const size_t rs = r16 ? R16_RS : n_grid;
+    const size_t fixed = (size_t)(n_grid + (r16 ? 256 : 0)) * sizeof(float2);
+    const int rc = (int)((budget - fixed) / (rs * sizeof(float2)));
+    return std::min(rc, 2 * IMG_MAX_KB);
+}
+
+static size_t img_lds_bytes(int n_grid, int rc_max, bool r16) {
+    const size_t rs = r16 ? R16_RS : n_grid;
+    return ((size_t)n_grid + (r16 ? 256 : 0) + (size_t)rc_max * rs) * sizeof(float2);
+}
+
+template <typename TOut, bool R16>
+static int launch_image(const ImgParams &P, int grid, size_t lds, hipStream_t s) {
+    (void)hipFuncSetAttribute((const void *)image_kernel<TOut, R16>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    image_kernel<TOut, R16><<<grid, IMG_THREADS, lds, s>>>(P);
+    return check_launch("dsx_image_subbands");
+}
+
 }  // namespace dsx
 
 using namespace dsx;
@@ -229,9 +348,8 @@ extern "C" int dsx_image_subbands(const int32_t *vis, const int32_t *tap_src,
     DSX_REQUIRE(img_dtype == DSX_F16 || img_dtype == DSX_F32, "dsx_image_subbands: img_dtype");
     int log2N = 0;
     while ((1 << log2N) < n_grid) ++log2N;
-    const size_t budget = 96 * 1024;
-    int rc_max = (int)((budget - (size_t)n_grid * sizeof(float2)) / ((size_t)n_grid * sizeof(float2)));
-    rc_max = std::min(rc_max, 2 * IMG_MAX_KB);
+    const bool r16 = n_grid == 256;
+    int rc_max = img_rows_per_chunk(n_grid, r16);
     rc_max = std::min(rc_max, n_rows);
     if (rc_max < 1) return set_error(DSX_ERANGE, "dsx_image_subbands: n_grid too large");
     const int n_chunks = (n_rows + rc_max - 1) / rc_max;
@@ -242,18 +360,11 @@ extern "C" int dsx_image_subbands(const int32_t *vis, const int32_t *tap_src,
     hipStream_t s = as_stream(stream);
     ImgParams P{vis, tap_src, tap_dst, tap_off, nullptr, img, n_steps, n_sub, per_sub, nbase, n_grid,
                 log2N, n_rows, rc_max, npad, scale, 0};
-    const size_t lds = (size_t)n_grid * sizeof(float2) * (1 + rc_max);
+    const size_t lds = img_lds_bytes(n_grid, rc_max, r16);
     const int grid = std::min(n_sub * n_steps, 4096);
-    if (img_dtype == DSX_F16) {
-        (void)hipFuncSetAttribute((const void *)image_kernel<_Float16>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        image_kernel<_Float16><<<grid, IMG_THREADS, lds, s>>>(P);
-    } else {
-        (void)hipFuncSetAttribute((const void *)image_kernel<float>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        image_kernel<float><<<grid, IMG_THREADS, lds, s>>>(P);
-    }
-    return check_launch("dsx_image_subbands");
+    if (img_dtype == DSX_F16)
+        return r16 ? launch_image<_Float16, true>(P, grid, lds, s) : launch_image<_Float16, false>(P, grid, lds, s);
+    return r16 ? launch_image<float, true>(P, grid, lds, s) : launch_image<float, false>(P, grid, lds, s);
 }
 
 extern "C" int dsx_grid_gather(const void *vis, const int32_t *cell_ptr, const int32_t *cell_src,
@@ -285,15 +396,15 @@ extern "C" int dsx_dirty_image(const void *grid, void *img, int n_images, int n_
     int log2N = 0;
     while ((1 << log2N) < n_grid) ++log2N;
     const int n_rows = n_grid / 2 + 1;
-    const size_t budget = 96 * 1024;
-    int rc_max = (int)((budget - (size_t)n_grid * sizeof(float2)) / ((size_t)n_grid * sizeof(float2)));
-    rc_max = std::min(std::min(rc_max, 2 * IMG_MAX_KB), n_rows);
+    const bool r16 = n_grid == 256;
+    int rc_max = std::min(img_rows_per_chunk(n_grid, r16), n_rows);
     if (rc_max < 1) return set_error(DSX_ERANGE, "dsx_dirty_image: n_grid too large");
     ImgParams P{nullptr, nullptr, nullptr, nullptr, (const float2 *)grid, img, n_images, 1, 1, 1,
                 n_grid, log2N, n_rows, rc_max, 0, prescale, 1};
-    const size_t lds = (size_t)n_grid * sizeof(float2) * (1 + rc_max);
-    (void)hipFuncSetAttribute((const void *)image_kernel<float>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    image_kernel<float><<<std::min(n_images, 4096), IMG_THREADS, lds, as_stream(stream)>>>(P);
-    return check_launch("dsx_dirty_image");
+    const size_t lds = img_lds_bytes(n_grid, rc_max, r16);
+    const int blocks = std::min(n_images, 4096);
+    const int rc = r16 ? launch_image<float, true>(P, blocks, lds, as_stream(stream))
+                       : launch_image<float, false>(P, blocks, lds, as_stream(stream));
+    if (rc != DSX_OK) return rc;
+    return DSX_OK;
 }

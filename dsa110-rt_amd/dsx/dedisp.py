@@ -194,15 +194,10 @@ class DedispPlan:
     r_total: int
     q_total: int
     paired: bool
+    lds_bytes: int            # LDS per work-group (windows, pair rows, lane term bases, table)
     sub_info: np.ndarray      # [n_sub, 3] int32
     rows: np.ndarray          # [n_rows, 6] int32
     tbase: np.ndarray         # [n_terms, n_dm] int32
-
-    @property
-    def lds_bytes(self) -> int:
-        return self.q_total * 4 + self.r_total * self._esize
-
-    _esize: int = 2
 
 
 def plan_dedisp(shift_sd: np.ndarray, *, n_widths: int, n_centres: int, ring_dtype: torch.dtype,
@@ -219,15 +214,15 @@ def plan_dedisp(shift_sd: np.ndarray, *, n_widths: int, n_centres: int, ring_dty
     sub_info = np.zeros((n_sub, 3), np.int32)
     rows = np.zeros((max(rows_cap, 1), 6), np.int32)
     tbase = np.zeros((max(n_sub, n_pair), n_dm), np.int32)
-    info = np.zeros(6, np.int32)
+    info = np.zeros(7, np.int32)
     P = _lib._P
     _lib.call("dsx_dedisp_plan", sh.ctypes.data_as(P), n_sub, n_dm, n_widths, n_centres, code, int(lds_budget),
               int(mode), sub_info.ctypes.data_as(P), rows.ctypes.data_as(P), rows_cap, tbase.ctypes.data_as(P),
               info.ctypes.data_as(P))
-    tile, n_rows, n_terms, r_total, q_total, paired = (int(x) for x in info)
+    tile, n_rows, n_terms, r_total, q_total, paired, lds = (int(x) for x in info)
     return DedispPlan(tile=tile, n_rows=n_rows, n_terms=n_terms, r_total=r_total, q_total=q_total,
-                      paired=bool(paired), sub_info=sub_info, rows=rows[:n_rows].copy(),
-                      tbase=tbase[:n_terms].copy(), _esize=2 if code == _lib.DSX_F16 else 4)
+                      paired=bool(paired), lds_bytes=lds, sub_info=sub_info, rows=rows[:n_rows].copy(),
+                      tbase=tbase[:n_terms].copy())
 
 
 def combine_chgroups(*, per_chgroup_streams: Mapping[int, torch.Tensor], time_shift_per_chgroup,

@@ -50,7 +50,7 @@ class SearchNode:
                  device: str = "cuda", ring_dtype: torch.dtype = torch.float16,
                  max_block_steps: Optional[int] = None, cand_cap: int = 1 << 20,
                  stream: Optional[torch.cuda.Stream] = None, lds_budget: int = LDS_BUDGET_DEDISP,
-                 dedisp_mode: int = 0):
+                 dedisp_mode: int = 2):
         cfg.validate()
         if ring_dtype not in (torch.float16, torch.float32):
             raise TypeError("ring_dtype must be float16 or float32")

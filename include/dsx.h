@@ -133,8 +133,9 @@ int dsx_ring_advance(const void *ring_old, void *ring_new, const void *img, int 
  *   rows     [rows_cap][6] pair rows (Q offset, length, R index of subband a
  *            at e = 0, R index of subband b at e = 0, b's window [lo, hi))
  *   tbase    [n_terms][n_dm] LDS index of each term of D(t) at t = tile start
- *   info[6]  = (tile, n_rows, n_terms, r_total, q_total, paired)
- * Pair rows: for subbands a = 2g, b = 2g + 1 and each distinct
+ *   info[7]  = (tile, n_rows, n_terms, r_total, q_total, paired, LDS bytes per
+ *              work-group)
+ * LDS holds float32 subband windows and pair rows.  Pair rows: for subbands a = 2g, b = 2g + 1 and each distinct
  * delta = shift[b][d] - shift[a][d], Q[tau] = ring_a[tau] + ring_b[tau - delta]
  * over the span of steps the DMs with that delta read; D(t) is then the sum
  * of n_sub / 2 rows instead of n_sub subbands. */
