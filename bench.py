@@ -33,6 +33,20 @@ MFMA_I8_PEAK_TOPS = 5000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
+def measured_traffic():
+    """HBM bytes per dedispersion launch from the committed hardware-counter
+    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "dedisp_c2_traffic.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,6 +55,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--form", choices=["chosen", "plain"], default="chosen")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (hardware-counter scale)")
     return ap.parse_args()
 
 
@@ -132,11 +148,19 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this one-GPU "
-              "build (the node pipeline shards by frequency/DM across GPUs, see DESIGN.md)",
+              "build (the node pipeline shards by frequency and DM across GPUs)",
               file=sys.stderr)
         sys.exit(2)
     from dsx import config as dcfg
     from dsx import synth
+    if args.calibrate_copy_mib > 0:
+        n = args.calibrate_copy_mib << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     cfg = dcfg.c2_config()
     bursts = synth.random_bursts(cfg, 8, 0xC2B, t_lo_step=cfg.n_steps,
                                  t_hi_step=(args.warmup + args.steps) * cfg.n_steps)
@@ -178,7 +202,7 @@ def main():
         "candidates": chosen["n_cands"],
         "roofline": {"kernel": "dsx_dedisp_detect (dedisp_kernel<_Float16, 6, false, 16>)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(),
                      "alg_bytes_per_launch": alg_bytes, "avg_ms": dd_ms,
                      "note": ("algorithmic bytes = fp16 subband images read once "
                               f"({ring_bytes}) + float64 mean/m2 read+write ({stat_bytes}); the "
