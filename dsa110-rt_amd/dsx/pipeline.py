@@ -50,7 +50,7 @@ class SearchNode:
                  device: str = "cuda", ring_dtype: torch.dtype = torch.float16,
                  max_block_steps: Optional[int] = None, cand_cap: int = 1 << 20,
                  stream: Optional[torch.cuda.Stream] = None, lds_budget: int = LDS_BUDGET_DEDISP,
-                 dedisp_mode: int = 2):
+                 dedisp_mode: int = 2, n_slots: int = 1):
         cfg.validate()
         if ring_dtype not in (torch.float16, torch.float32):
             raise TypeError("ring_dtype must be float16 or float32")
@@ -86,8 +86,16 @@ class SearchNode:
         self.cap = int(max_block_steps or cfg.n_steps)
         # ---- buffers
         nb = cfg.nbase
-        self.vis = torch.empty((self.cap, cfg.n_chan_out, nb, 2), dtype=torch.int32, device=self.dev)
-        self.img = torch.empty((cfg.n_sub, self.cap, N, N), dtype=ring_dtype, device=self.dev)
+        # per-slot block buffers: a block's front stage (correlate + image) can run
+        # while the previous block's back stage (ring + dedispersion) still reads its slot
+        if n_slots < 1:
+            raise ValueError("n_slots must be >= 1")
+        self.n_slots = int(n_slots)
+        self.vis_slots = [torch.empty((self.cap, cfg.n_chan_out, nb, 2), dtype=torch.int32, device=self.dev)
+                          for _ in range(self.n_slots)]
+        self.img_slots = [torch.empty(cfg.n_sub * self.cap * N * N, dtype=ring_dtype, device=self.dev)
+                          for _ in range(self.n_slots)]
+        self.vis = self.vis_slots[0]
         row_len = self.hist + self.cap
         self.row_len = row_len
         self.rings = [torch.zeros((cfg.n_sub, self.n_pix, row_len), dtype=ring_dtype, device=self.dev)
@@ -99,8 +107,16 @@ class SearchNode:
         self.n_prev = torch.zeros(cfg.n_dm, dtype=torch.float64, device=self.dev)
         self.blk_count = torch.zeros(cfg.n_dm, dtype=torch.float64, device=self.dev)
         self.cand_cap = int(cand_cap)
-        self.cand = torch.empty((self.cand_cap, 4), dtype=torch.int32, device=self.dev)
-        self.n_cand = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        self.cand_slots = [torch.empty((self.cand_cap, 4), dtype=torch.int32, device=self.dev)
+                           for _ in range(self.n_slots)]
+        self.n_cand_slots = [torch.zeros(1, dtype=torch.int32, device=self.dev) for _ in range(self.n_slots)]
+        self.cand, self.n_cand = self.cand_slots[0], self.n_cand_slots[0]
+        # pinned host staging per slot: block counts in, candidate count + records out
+        self._rec_prefix = min(self.cand_cap, 1 << 16)
+        self._blk_pinned = [torch.zeros(cfg.n_dm, dtype=torch.float64, pin_memory=True) for _ in range(self.n_slots)]
+        self._cnt_pinned = [torch.zeros(1, dtype=torch.int32, pin_memory=True) for _ in range(self.n_slots)]
+        self._rec_pinned = [torch.zeros((self._rec_prefix, 4), dtype=torch.int32, pin_memory=True)
+                            for _ in range(self.n_slots)]
         self.lds_budget = int(lds_budget)
         self.dedisp_mode = int(dedisp_mode)
         self.plans: Dict[int, tuple] = {}
@@ -145,85 +161,154 @@ class SearchNode:
         return np.clip(c_lo + L - start, 0, L).astype(np.float64)
 
     # ------------------------------------------------------------------ stages
-    def correlate(self, raw: torch.Tensor, n_samp: int) -> torch.Tensor:
+    def _stream(self, stream: Optional[torch.cuda.Stream]) -> torch.cuda.Stream:
+        if stream is not None:
+            return stream
+        return self.stream if self.stream is not None else torch.cuda.current_stream()
+
+    def correlate(self, raw: torch.Tensor, n_samp: int, slot: int = 0,
+                  stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
         cfg = self.cfg
         n_pkt = n_samp // 2
         n_steps = n_samp // cfg.t_int
         _lib.require_gpu(raw)
         if raw.dtype != torch.uint8 or raw.numel() != n_pkt * cfg.n_ant * cfg.n_chan * 4:
             raise ValueError("raw must be uint8 of shape (n_pkt, n_ant, n_chan, 2, 2)")
-        vis = self.vis[:n_steps]
+        if n_steps > self.cap:
+            raise ValueError(f"{n_steps} steps > max_block_steps {self.cap}")
+        vis = self.vis_slots[slot][:n_steps]
         self._mark("corr", 0)
         _lib.call("dsx_corr_i8", _lib.ptr(raw), None, _lib.ptr(vis), n_pkt, cfg.n_ant, cfg.n_chan,
-                  cfg.t_int, cfg.csf, 1, 0, 1, _lib.stream(self.stream))
+                  cfg.t_int, cfg.csf, 1, 0, 1, _lib.stream(self._stream(stream)))
         self._mark("corr", 1)
         return vis
 
-    def image(self, vis: torch.Tensor, n_steps: int) -> torch.Tensor:
+    def image(self, vis: torch.Tensor, n_steps: int, slot: int = 0,
+              stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
         cfg = self.cfg
-        img = self.img[:, :n_steps] if n_steps == self.cap else \
-            torch.empty((cfg.n_sub, n_steps, cfg.n_grid, cfg.n_grid), dtype=self.ring_dtype, device=self.dev)
+        N = cfg.n_grid
+        img = self.img_slots[slot][:cfg.n_sub * n_steps * N * N].view(cfg.n_sub, n_steps, N, N)
         self._mark("image", 0)
         _lib.call("dsx_image_subbands", _lib.ptr(vis), _lib.ptr(self.tap_src), _lib.ptr(self.tap_dst),
                   _lib.ptr(self.tap_off), _lib.ptr(img), n_steps, cfg.n_sub, cfg.per_sub, cfg.nbase,
                   cfg.n_grid, self.n_rows, self.npad, 1.0,
                   _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
-                  _lib.stream(self.stream))
+                  _lib.stream(self._stream(stream)))
         self._mark("image", 1)
         return img
 
-    def push_images(self, img: torch.Tensor, n_steps: int) -> None:
+    def push_images(self, img: torch.Tensor, n_steps: int, stream: Optional[torch.cuda.Stream] = None) -> None:
         cfg = self.cfg
+        if n_steps > self.cap:
+            raise ValueError(f"{n_steps} steps > max_block_steps {self.cap}")
+        _lib.require_gpu(img)
         old, new = self.rings[self.cur], self.rings[1 - self.cur]
         self._mark("ring", 0)
         _lib.call("dsx_ring_advance", _lib.ptr(old), _lib.ptr(new), _lib.ptr(img), cfg.n_sub,
                   self.n_pix, self.hist, self.cap, self.prev_len, n_steps,
                   _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
-                  _lib.stream(self.stream))
+                  _lib.stream(self._stream(stream)))
         self._mark("ring", 1)
         self.cur = 1 - self.cur
         self.prev_len = n_steps
 
-    def detect(self, n_steps: int) -> BlockResult:
-        """Dedisperse + detect the block just pushed (centres [T0 - 64, T0 + L - 64))."""
+    def detect_async(self, n_steps: int, slot: int = 0,
+                     stream: Optional[torch.cuda.Stream] = None) -> "PendingBlock":
+        """Enqueue dedispersion + detection of the block just pushed (centres
+        [T0 - 64, T0 + L - 64)) and the copy of its candidate records to host
+        memory; :meth:`finish` waits for them and decodes."""
         cfg = self.cfg
+        s = self._stream(stream)
         T0 = self.T0
         c_lo = T0 - DETECT_DELAY_STEPS
         plan, pd = self._plan_for(n_steps)
         counts = self._block_counts(c_lo, n_steps)
-        self.blk_count.copy_(torch.from_numpy(counts), non_blocking=False)
-        self.n_cand.zero_()
-        self._mark("dedisp", 0)
-        _lib.call("dsx_dedisp_detect", _lib.ptr(self.rings[self.cur]),
-                  _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
-                  cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(pd["sub_info"]),
-                  _lib.ptr(pd["rows"]) if plan.n_rows else None, plan.n_rows, _lib.ptr(pd["tbase"]),
-                  plan.n_terms, cfg.n_dm, _lib.ptr(self.valid_from), _lib.ptr(self.n_prev),
-                  _lib.ptr(self.blk_count), _lib.ptr(self.mean), _lib.ptr(self.m2), c_lo, n_steps,
-                  self.n_widths, plan.tile, plan.r_total, plan.q_total, float(cfg.threshold),
-                  float(cfg.min_stat_count),
-                  _lib.ptr(self.cand), self.cand_cap, _lib.ptr(self.n_cand), _lib.stream(self.stream))
-        self._mark("dedisp", 1)
-        n = int(self.n_cand.item())
+        cand, n_cand = self.cand_slots[slot], self.n_cand_slots[slot]
+        with torch.cuda.stream(s):
+            self._blk_pinned[slot].copy_(torch.from_numpy(counts))
+            self.blk_count.copy_(self._blk_pinned[slot], non_blocking=True)
+            n_cand.zero_()
+            self._mark("dedisp", 0)
+            _lib.call("dsx_dedisp_detect", _lib.ptr(self.rings[self.cur]),
+                      _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
+                      cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(pd["sub_info"]),
+                      _lib.ptr(pd["rows"]) if plan.n_rows else None, plan.n_rows, _lib.ptr(pd["tbase"]),
+                      plan.n_terms, cfg.n_dm, _lib.ptr(self.valid_from), _lib.ptr(self.n_prev),
+                      _lib.ptr(self.blk_count), _lib.ptr(self.mean), _lib.ptr(self.m2), c_lo, n_steps,
+                      self.n_widths, plan.tile, plan.r_total, plan.q_total, float(cfg.threshold),
+                      float(cfg.min_stat_count), _lib.ptr(cand), self.cand_cap, _lib.ptr(n_cand),
+                      _lib.stream(s))
+            self._mark("dedisp", 1)
+            self.n_prev += self.blk_count
+            self._cnt_pinned[slot].copy_(n_cand, non_blocking=True)
+            self._rec_pinned[slot].copy_(cand[:self._rec_prefix], non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(s)
+        self.n_prev_host += counts
+        self.T0 = T0 + n_steps
+        return PendingBlock(slot=slot, T0=T0, n_steps=n_steps, done=done, stream=s)
+
+    def finish(self, pend: "PendingBlock") -> BlockResult:
+        """Wait for a block's detection and decode its candidates."""
+        cfg = self.cfg
+        pend.done.synchronize()
+        n = int(self._cnt_pinned[pend.slot][0])
         k = min(n, self.cand_cap)
-        rec = self.cand[:k].cpu().numpy().view(CAND_DTYPE).reshape(-1) if k else np.zeros(0, CAND_DTYPE)
+        if k <= self._rec_prefix:
+            rec = self._rec_pinned[pend.slot][:k].numpy().copy().view(CAND_DTYPE).reshape(-1)
+        else:
+            with torch.cuda.stream(pend.stream):
+                rec = self.cand_slots[pend.slot][:k].cpu().numpy().view(CAND_DTYPE).reshape(-1)
         cands = decode_chunk_maxima(rec, widths=cfg.widths, n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
                                     threshold=cfg.threshold, n_top=cfg.n_top, radius_t=cfg.radius_t,
                                     radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm, t_int=cfg.t_int)
-        self.n_prev_host += counts
-        self.n_prev.copy_(torch.from_numpy(self.n_prev_host))
-        self.T0 = T0 + n_steps
-        return BlockResult(candidates=cands, n_raw=n, overflow=n > self.cand_cap, T0=T0, n_steps=n_steps,
-                           records=rec)
+        return BlockResult(candidates=cands, n_raw=n, overflow=n > self.cand_cap, T0=pend.T0,
+                           n_steps=pend.n_steps, records=rec)
+
+    def detect(self, n_steps: int) -> BlockResult:
+        """Dedisperse + detect the block just pushed and wait for its candidates."""
+        return self.finish(self.detect_async(n_steps))
+
+    def _check_block(self, n_samp: int) -> int:
+        cfg = self.cfg
+        n_steps = n_samp // cfg.t_int
+        if n_samp % (2 * cfg.t_int) or n_steps % DETECT_CHUNK_STEPS or n_steps > self.cap or n_steps <= 0:
+            raise ValueError(f"block of {n_samp} samples: steps must be a positive multiple of "
+                             f"{DETECT_CHUNK_STEPS} and <= {self.cap}")
+        return n_steps
+
+    def front(self, raw: torch.Tensor, n_samp: int, slot: int = 0,
+              stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
+        """Correlate and image a block into ``slot``; returns its images."""
+        n_steps = self._check_block(n_samp)
+        vis = self.correlate(raw, n_samp, slot, stream)
+        return self.image(vis, n_steps, slot, stream)
+
+    def back(self, img: torch.Tensor, n_steps: int, slot: int = 0,
+             stream: Optional[torch.cuda.Stream] = None) -> "PendingBlock":
+        """Push a block's images into the history and enqueue its detection.
+        ``pend.img_free`` marks when the slot's images have been consumed."""
+        s = self._stream(stream)
+        self.push_images(img, n_steps, s)
+        img_free = torch.cuda.Event()
+        img_free.record(s)
+        pend = self.detect_async(n_steps, slot, s)
+        pend.img_free = img_free
+        return pend
 
     def process_block(self, raw: torch.Tensor, n_samp: Optional[int] = None) -> BlockResult:
-        cfg = self.cfg
-        n_samp = cfg.n_samp if n_samp is None else int(n_samp)
-        n_steps = n_samp // cfg.t_int
-        if n_samp % (2 * cfg.t_int) or n_steps % DETECT_CHUNK_STEPS or n_steps > self.cap:
-            raise ValueError(f"block of {n_samp} samples: steps must be a multiple of "
-                             f"{DETECT_CHUNK_STEPS} and <= {self.cap}")
-        vis = self.correlate(raw, n_samp)
-        img = self.image(vis, n_steps)
-        self.push_images(img, n_steps)
-        return self.detect(n_steps)
+        n_samp = self.cfg.n_samp if n_samp is None else int(n_samp)
+        n_steps = self._check_block(n_samp)
+        img = self.front(raw, n_samp)
+        return self.finish(self.back(img, n_steps))
+
+
+@dataclass
+class PendingBlock:
+    """A block whose detection is enqueued (see :meth:`SearchNode.finish`)."""
+    slot: int
+    T0: int
+    n_steps: int
+    done: torch.cuda.Event
+    stream: torch.cuda.Stream
+    img_free: Optional[torch.cuda.Event] = None
