@@ -46,6 +46,26 @@ __global__ __launch_bounds__(256) void ring_transpose_kernel(const T *img, T *ri
     }
 }
 
+// out[s][t][k] = sum_{j < f} in[s][f t + j][k]: f-fold time downsampling of
+// (n_planes, n_in, n_inner) planes, float32 accumulation, one thread per output
+// element (coalesced along k).
+template <typename T>
+__global__ __launch_bounds__(256) void downsample_time_kernel(const T *in, T *out, int64_t n_out_total,
+                                                              int n_in, int64_t n_inner, int f) {
+    const int n_out = n_in / f;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n_out_total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = g % n_inner;
+        const int64_t rest = g / n_inner;
+        const int t = (int)(rest % n_out);
+        const int64_t s = rest / n_out;
+        const T *src = in + (s * n_in + (int64_t)f * t) * n_inner + k;
+        float acc = 0.f;
+        for (int j = 0; j < f; ++j) acc += ld_as_float(src, (int64_t)j * n_inner);
+        st_from_float(out, g, acc);
+    }
+}
+
 }  // namespace dsx
 
 using namespace dsx;
@@ -80,4 +100,24 @@ extern "C" int dsx_ring_advance(const void *ring_old, void *ring_new, const void
                                                          n_pix, n_new, hist, row_len);
     }
     return check_launch("dsx_ring_advance");
+}
+
+extern "C" int dsx_downsample_time(const void *in, void *out, int n_planes, int n_in, int64_t n_inner,
+                                   int factor, int dtype, dsx_stream_t stream) {
+    DSX_REQUIRE(in && out, "dsx_downsample_time: null pointer");
+    DSX_REQUIRE(n_planes > 0 && n_in > 0 && n_inner > 0, "dsx_downsample_time: bad shape");
+    DSX_REQUIRE(factor >= 1 && n_in % factor == 0, "dsx_downsample_time: factor %d must divide n_in %d",
+                factor, n_in);
+    DSX_REQUIRE(in != out, "dsx_downsample_time: in and out must differ");
+    DSX_REQUIRE(dtype == DSX_F16 || dtype == DSX_F32, "dsx_downsample_time: dtype");
+    const int64_t total = (int64_t)n_planes * (n_in / factor) * n_inner;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 1 << 20);
+    hipStream_t s = as_stream(stream);
+    if (dtype == DSX_F16)
+        downsample_time_kernel<_Float16><<<blocks, 256, 0, s>>>((const _Float16 *)in, (_Float16 *)out, total,
+                                                                 n_in, n_inner, factor);
+    else
+        downsample_time_kernel<float><<<blocks, 256, 0, s>>>((const float *)in, (float *)out, total, n_in,
+                                                              n_inner, factor);
+    return check_launch("dsx_downsample_time");
 }

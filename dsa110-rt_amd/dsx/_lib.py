@@ -32,6 +32,7 @@ PROTOTYPES = {
     "dsx_image_subbands": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _F, _I, _P],
     "dsx_dirty_image": [_P, _P, _I, _I, _F, _P],
     "dsx_ring_advance": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],
+    "dsx_downsample_time": [_P, _P, _I, _I, _I64, _I, _I, _P],
     "dsx_dedisp_plan": [_P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P],
     "dsx_dedisp_detect": [_P, _I, _I, _I, _I, _I, _I64, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P,
                           _I64, _I, _I, _I, _I, _I, _F, _D, _P, _I, _P, _P],

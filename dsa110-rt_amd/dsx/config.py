@@ -43,6 +43,10 @@ class NodeConfig:
     cell_lambda: Optional[float] = None    # None: auto-fit to the longest baseline
     dm_max: float = 500.0
     n_dm: int = 64
+    dm_min: float = 0.0
+    dm_endpoint: bool = True               # trials include dm_max
+    # uneven plan: ((dm_lo, dm_hi, n_dm, time downsampling, endpoint), ...); empty = one range
+    dm_ranges: Tuple[Tuple[float, float, int, int, bool], ...] = ()
     widths: Tuple[int, ...] = (1, 2, 4, 8)
     threshold: float = 8.0
     min_stat_count: int = 512              # complete samples per DM before it is scored
@@ -122,7 +126,26 @@ class NodeConfig:
 
     @property
     def dm_trials(self) -> np.ndarray:
-        return np.linspace(0.0, float(self.dm_max), int(self.n_dm))
+        """DM trials of the node: the concatenated ranges of an uneven plan, else
+        ``n_dm`` evenly spaced values from ``dm_min`` (to ``dm_max`` when
+        ``dm_endpoint``)."""
+        if self.dm_ranges:
+            return np.concatenate([r.dm_trials for r in self.range_configs()])
+        return np.linspace(float(self.dm_min), float(self.dm_max), int(self.n_dm), endpoint=self.dm_endpoint)
+
+    def range_configs(self) -> Tuple["NodeConfig", ...]:
+        """One single-range configuration per DM range of an uneven plan: the
+        range's trials, its time resolution (``t_int`` times the downsampling)
+        and the node's widths in samples of that resolution."""
+        out = []
+        for lo, hi, n, f, endpoint in self.dm_ranges:
+            out.append(replace(self, t_int=self.t_int * int(f), dm_min=float(lo), dm_max=float(hi), n_dm=int(n),
+                               dm_endpoint=bool(endpoint), dm_ranges=()))
+        return tuple(out)
+
+    @property
+    def range_factors(self) -> Tuple[int, ...]:
+        return tuple(int(r[3]) for r in self.dm_ranges) if self.dm_ranges else (1,)
 
     def shift_table(self) -> np.ndarray:
         return subband_shift_table(self.dm_trials, self.subband_ref_GHz,
@@ -159,6 +182,12 @@ class NodeConfig:
             raise ValueError("widths must be 1, 2, 4, ... without gaps")
         if self.w_max > 2 * DETECT_DELAY_STEPS:
             raise ValueError("widest boxcar exceeds twice the detection delay")
+        for lo, hi, n, f, _ in self.dm_ranges:
+            if int(f) < 1 or self.n_steps % (DETECT_CHUNK_STEPS * int(f)) != 0:
+                raise ValueError(f"DM range {lo}-{hi}: downsampling {f} must leave a multiple of "
+                                 f"{DETECT_CHUNK_STEPS} steps per block")
+            if int(n) < 1 or not hi > lo:
+                raise ValueError(f"DM range {lo}-{hi} with {n} trials")
 
     def with_(self, **kw) -> "NodeConfig":
         return replace(self, **kw)
@@ -185,6 +214,15 @@ def c2_config() -> NodeConfig:
         dm_max=1500.0, n_dm=256, widths=(1, 2, 4, 8, 16, 32), threshold=8.0,
         layout_seed=0xD5A11096, volt_seed=0xD5A11097,
     )
+
+
+def c4_config() -> NodeConfig:
+    """Config-2 inputs with the uneven DM plan: 0-300 (128 trials, full time
+    resolution), 300-800 (96 trials, 2x downsampled), 800-1500 (64 trials, 4x
+    downsampled); boxcars 1-64 samples of each range's resolution."""
+    return c2_config().with_(
+        name="c4_96ant_384ch_uneven", widths=(1, 2, 4, 8, 16, 32, 64), n_dm=288, dm_max=1500.0,
+        dm_ranges=((0.0, 300.0, 128, 1, False), (300.0, 800.0, 96, 2, False), (800.0, 1500.0, 64, 4, True)))
 
 
 def c2_small_config(n_samp: int = 1024) -> NodeConfig:

@@ -50,7 +50,7 @@ class SearchNode:
                  device: str = "cuda", ring_dtype: torch.dtype = torch.float16,
                  max_block_steps: Optional[int] = None, cand_cap: int = 1 << 20,
                  stream: Optional[torch.cuda.Stream] = None, lds_budget: int = LDS_BUDGET_DEDISP,
-                 dedisp_mode: int = 2, n_slots: int = 1):
+                 dedisp_mode: int = 2, n_slots: int = 1, with_front: bool = True, with_detect: bool = True):
         cfg.validate()
         if ring_dtype not in (torch.float16, torch.float32):
             raise TypeError("ring_dtype must be float16 or float32")
@@ -91,24 +91,27 @@ class SearchNode:
         if n_slots < 1:
             raise ValueError("n_slots must be >= 1")
         self.n_slots = int(n_slots)
+        nf = self.n_slots if with_front else 0
         self.vis_slots = [torch.empty((self.cap, cfg.n_chan_out, nb, 2), dtype=torch.int32, device=self.dev)
-                          for _ in range(self.n_slots)]
+                          for _ in range(nf)]
         self.img_slots = [torch.empty(cfg.n_sub * self.cap * N * N, dtype=ring_dtype, device=self.dev)
-                          for _ in range(self.n_slots)]
-        self.vis = self.vis_slots[0]
+                          for _ in range(nf)]
+        self.vis = self.vis_slots[0] if nf else None
         row_len = self.hist + self.cap
         self.row_len = row_len
+        nd = 2 if with_detect else 0
         self.rings = [torch.zeros((cfg.n_sub, self.n_pix, row_len), dtype=ring_dtype, device=self.dev)
-                      for _ in range(2)]
+                      for _ in range(nd)]
         self.cur = 0
-        self.mean = torch.zeros((self.n_pix, cfg.n_dm), dtype=torch.float64, device=self.dev)
-        self.m2 = torch.zeros((self.n_pix, cfg.n_dm), dtype=torch.float64, device=self.dev)
+        ns = self.n_pix if with_detect else 0
+        self.mean = torch.zeros((ns, cfg.n_dm), dtype=torch.float64, device=self.dev)
+        self.m2 = torch.zeros((ns, cfg.n_dm), dtype=torch.float64, device=self.dev)
         self.n_prev_host = np.zeros(cfg.n_dm, dtype=np.float64)
         self.n_prev = torch.zeros(cfg.n_dm, dtype=torch.float64, device=self.dev)
         self.blk_count = torch.zeros(cfg.n_dm, dtype=torch.float64, device=self.dev)
         self.cand_cap = int(cand_cap)
-        self.cand_slots = [torch.empty((self.cand_cap, 4), dtype=torch.int32, device=self.dev)
-                           for _ in range(self.n_slots)]
+        self.cand_slots = [torch.empty((self.cand_cap if with_detect else 0, 4), dtype=torch.int32,
+                                       device=self.dev) for _ in range(self.n_slots)]
         self.n_cand_slots = [torch.zeros(1, dtype=torch.int32, device=self.dev) for _ in range(self.n_slots)]
         self.cand, self.n_cand = self.cand_slots[0], self.n_cand_slots[0]
         # pinned host staging per slot: block counts in, candidate count + records out
@@ -312,3 +315,74 @@ class PendingBlock:
     done: torch.cuda.Event
     stream: torch.cuda.Stream
     img_free: Optional[torch.cuda.Event] = None
+
+
+class MultiRangeNode:
+    """Node stream with an uneven DM plan (``cfg.dm_ranges``): one front stage
+    (correlate + image at full time resolution) feeding one image history and
+    detector per DM range, each at its own time resolution (images summed over
+    ``f`` consecutive steps by ``dsx_downsample_time``) with its own trials,
+    widths in samples of that resolution, float64 statistics and LDS plan.
+    Candidates of all ranges are decoded together in full-resolution steps:
+    time ``t * f``, width ``w * f``, DM index offset by the earlier ranges'
+    trials."""
+
+    def __init__(self, cfg: NodeConfig, antpos_e: np.ndarray, antpos_n: np.ndarray, *,
+                 device: str = "cuda", ring_dtype: torch.dtype = torch.float16,
+                 max_block_steps: Optional[int] = None, stream: Optional[torch.cuda.Stream] = None):
+        if not cfg.dm_ranges:
+            raise ValueError("MultiRangeNode needs cfg.dm_ranges")
+        cfg.validate()
+        self.cfg = cfg
+        self.stream = stream
+        cap = int(max_block_steps or cfg.n_steps)
+        self.front_node = SearchNode(cfg.with_(dm_ranges=()), antpos_e, antpos_n, device=device,
+                                     ring_dtype=ring_dtype, max_block_steps=cap, stream=stream, with_detect=False)
+        self.ranges = []
+        off = 0
+        for rc, (lo, hi, n, f, _) in zip(cfg.range_configs(), cfg.dm_ranges):
+            nd = SearchNode(rc, antpos_e, antpos_n, device=device, ring_dtype=ring_dtype,
+                            max_block_steps=cap // int(f), stream=stream, with_front=False)
+            buf = None if int(f) == 1 else torch.empty(cfg.n_sub * (cap // int(f)) * cfg.n_grid ** 2,
+                                                       dtype=ring_dtype, device=torch.device(device))
+            self.ranges.append((nd, int(f), off, buf))
+            off += int(n)
+        self.widths_all = [w * f for _, f, _, _ in self.ranges for w in cfg.widths]
+        self.ring_dtype = ring_dtype
+
+    def process_block(self, raw: torch.Tensor, n_samp: Optional[int] = None) -> BlockResult:
+        cfg = self.cfg
+        n_samp = cfg.n_samp if n_samp is None else int(n_samp)
+        n_steps = self.front_node._check_block(n_samp)
+        s = self.front_node._stream(None)
+        img = self.front_node.front(raw, n_samp, 0, s)
+        N2 = cfg.n_grid * cfg.n_grid
+        pends = []
+        for nd, f, off, buf in self.ranges:
+            if n_steps % (DETECT_CHUNK_STEPS * f):
+                raise ValueError(f"{n_steps} steps do not downsample by {f} into whole 64-sample chunks")
+            if f == 1:
+                img_f = img
+            else:
+                img_f = buf[:cfg.n_sub * (n_steps // f) * N2].view(cfg.n_sub, n_steps // f, cfg.n_grid, cfg.n_grid)
+                _lib.call("dsx_downsample_time", _lib.ptr(img), _lib.ptr(img_f), cfg.n_sub, n_steps, N2, f,
+                          _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32, _lib.stream(s))
+            nd.push_images(img_f, n_steps // f, s)
+            pends.append(nd.detect_async(n_steps // f, 0, s))
+        recs, T0 = [], None
+        k_off = 0
+        for (nd, f, off, _), pend in zip(self.ranges, pends):
+            res = nd.finish(pend)
+            T0 = pend.T0 * f if T0 is None else T0
+            r = res.records.copy()
+            r["t"] = r["t"] * f
+            r["d"] = r["d"] + off
+            r["k"] = r["k"] + k_off
+            recs.append(r)
+            k_off += len(cfg.widths)
+        rec = np.concatenate(recs) if recs else np.zeros(0, CAND_DTYPE)
+        cands = decode_chunk_maxima(rec, widths=self.widths_all, n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
+                                    threshold=cfg.threshold, n_top=cfg.n_top, radius_t=cfg.radius_t,
+                                    radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm, t_int=cfg.t_int)
+        return BlockResult(candidates=cands, n_raw=int(rec.size), overflow=False, T0=int(T0), n_steps=n_steps,
+                           records=rec)

@@ -123,6 +123,15 @@ int dsx_ring_advance(const void *ring_old, void *ring_new, const void *img, int 
                      int n_pix, int hist, int cap, int prev_len, int n_new, int dtype,
                      dsx_stream_t stream);
 
+/* f-fold time downsampling of (n_planes, n_in, n_inner) planes:
+ * out[s][t][k] = sum_{j<f} in[s][f t + j][k], t < n_in / f (float32 sums,
+ * dtype DSX_F16 or DSX_F32 for both).  Feeds the coarser-resolution image
+ * histories of an uneven DM plan.
+ * Same sum as the reference's float64 decimate(), src/dsart/x2/ref/corr2x_ref.py:234-237
+ * (D consecutive samples -> one stream sample). */
+int dsx_downsample_time(const void *in, void *out, int n_planes, int n_in, int64_t n_inner,
+                        int factor, int dtype, dsx_stream_t stream);
+
 /* ------------------------------------------------ dedispersion + detector
  * Layout of the LDS time windows for a DM plan (host function, no GPU work).
  * host_shift is [s][d] (host memory).  Picks the largest time tile (multiple

@@ -39,7 +39,10 @@ class OracleStream:
         per = nco // cfg.n_sub
         nu_ref = self.freq_out[::per][: cfg.n_sub]
         t_step = cfg.t_int * osearch.NATIVE_US
-        self.dm = np.linspace(0.0, float(cfg.dm_max), int(cfg.n_dm))
+        # DM trial values are configuration (an explicit list when the config
+        # carries one, e.g. one range of an uneven plan)
+        self.dm = (np.asarray(cfg.dm_trials, dtype=np.float64) if hasattr(cfg, "dm_trials")
+                   else np.linspace(0.0, float(cfg.dm_max), int(cfg.n_dm)))
         self.shifts = osearch.shift_table(self.dm, nu_ref, float(f[-1]), t_step)
         self.mask = oimage.edge_mask(cfg.n_grid, 1)
         n_pix_full = cfg.n_grid * cfg.n_grid
