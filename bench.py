@@ -36,7 +36,7 @@ FP32_PEAK_TFLOPS = 157.3
 def measured_traffic():
     """HBM bytes per dedispersion launch from the committed hardware-counter
     summary (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", "dedisp_c2_traffic.md")
+    path = os.path.join(ROOT, "profiles", "dedisp_traffic_96ant_384ch_16384.md")
     try:
         with open(path) as fh:
             for line in fh:
@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--form", choices=["chosen", "plain"], default="chosen")
     ap.add_argument("--dm-plan", choices=["even", "uneven"], default="even",
-                    help="uneven: config-2 inputs with the three-range DM plan (1x/2x/4x, widths 1-64)")
+                    help="uneven: 96-antenna 384-channel inputs with the three-range DM plan (1x/2x/4x, widths 1-64)")
     ap.add_argument("--serve", action="store_true",
                     help="serving variant: uneven blocks, Poisson arrivals, HBM-budget admission")
     ap.add_argument("--serve-blocks", type=int, default=200)
@@ -86,7 +86,10 @@ def run_form(cfg, sc, blocks, steps, warmup, ring_dtype, plain=False):
         node = SearchNode(cfg, sc.antpos_e, sc.antpos_n, ring_dtype=ring_dtype)
     for i in range(warmup):
         node.process_block(blocks[i % len(blocks)])
+    tune = None
     if not plain:
+        # LDS layout (time tile, pair rows or not) picked by timing each candidate
+        tune = node.autotune(node._check_block(cfg.n_samp))
         node.enable_timing()
     torch.cuda.synchronize()
     lat, stages, n_cands = [], {}, 0
@@ -103,7 +106,7 @@ def run_form(cfg, sc, blocks, steps, warmup, ring_dtype, plain=False):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     return dict(wall=wall, lat=np.asarray(lat), stage_ms={k: float(np.mean(v)) for k, v in stages.items()},
-                n_cands=n_cands, node=node)
+                n_cands=n_cands, node=node, tune=tune)
 
 
 def cpu_baseline(cfg, sc):
@@ -159,7 +162,7 @@ def serve_main(args):
     from dsx import synth
     from dsx.pipeline import SearchNode
     from dsx.serve import BlockRequest, NodeServer, poisson_arrivals, summarize
-    cfg = dcfg.c2_config()
+    cfg = dcfg.search_node_config()
     rng = np.random.default_rng(0xC5)
     lens = [int(x) for x in rng.choice([4096, 8192, 16384, 32768], size=args.serve_blocks)]
     starts = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
@@ -198,12 +201,12 @@ def serve_main(args):
 
 
 def uneven_main(args):
-    """Config-2 inputs through the uneven DM plan (0-300: 128 trials at full
+    """96-antenna 384-channel inputs through the uneven DM plan (0-300: 128 trials at full
     resolution; 300-800: 96 at 2x; 800-1500: 64 at 4x; widths 1-64)."""
     from dsx import config as dcfg
     from dsx import synth
     from dsx.pipeline import MultiRangeNode
-    cfg = dcfg.c4_config()
+    cfg = dcfg.uneven_dm_config()
     bursts = synth.random_bursts(cfg.with_(dm_ranges=()), 8, 0xC4B, t_lo_step=cfg.n_steps,
                                  t_hi_step=(args.warmup + args.steps) * cfg.n_steps)
     sc = synth.make_scenario(cfg.with_(dm_ranges=()), bursts)
@@ -212,6 +215,7 @@ def uneven_main(args):
     node = MultiRangeNode(cfg, sc.antpos_e, sc.antpos_n)
     for i in range(args.warmup):
         node.process_block(blocks[i])
+    tune = node.autotune()
     torch.cuda.synchronize()
     lat, n_c = [], 0
     t_start = time.perf_counter()
@@ -228,10 +232,12 @@ def uneven_main(args):
         "value": args.steps * cfg.block_seconds / wall, "unit": "x real-time", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True, "data": "synthetic (seeded 4-bit noise + 8 dispersed bursts)",
-        "config": {"workload": "config-2 inputs; DM 0-300 x128 @1x, 300-800 x96 @2x, 800-1500 x64 @4x; "
+        "config": {"workload": "96-antenna 384-channel inputs; DM 0-300 x128 @1x, 300-800 x96 @2x, 800-1500 x64 @4x; "
                                "boxcars 1-64 samples of each range's resolution"},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3),
                        "block_duration": cfg.block_seconds * 1e3},
+        "dedisp_layouts": [{"tile": nd.plans[cfg.n_steps // f][0].tile, "paired": nd.plans[cfg.n_steps // f][0].paired,
+                            "autotune_ms": t} for (nd, f, _, _), t in zip(node.ranges, tune)],
         "candidates": n_c}), flush=True)
 
 
@@ -258,7 +264,7 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    cfg = dcfg.c2_config()
+    cfg = dcfg.search_node_config()
     bursts = synth.random_bursts(cfg, 8, 0xC2B, t_lo_step=cfg.n_steps,
                                  t_hi_step=(args.warmup + args.steps) * cfg.n_steps)
     sc = synth.make_scenario(cfg, bursts)
@@ -279,6 +285,9 @@ def main():
     stat_bytes = 2 * 2 * n_pix * cfg.n_dm * 8
     alg_bytes = ring_bytes + stat_bytes
     achieved = alg_bytes / (dd_ms * 1e-3) / 1e9
+    plan = node.plans[cfg.n_steps][0]
+    maxt = 8 if plan.n_terms <= 8 else 16 if plan.n_terms <= 16 else 0
+    kname = f"dedisp_kernel<_Float16, {len(cfg.widths)}, {str(plan.paired).lower()}, {maxt}>"
     out = {
         "metric": "real-time factor + p99 block->candidate latency, 96-ant 384-ch node; chosen vs plain",
         "value": rtf,
@@ -297,7 +306,9 @@ def main():
                        "block_duration": cfg.block_seconds * 1e3},
         "stages_ms": chosen["stage_ms"],
         "candidates": chosen["n_cands"],
-        "roofline": {"kernel": "dsx_dedisp_detect (dedisp_kernel<_Float16, 6, false, 16>)", "bound": "hbm",
+        "dedisp_layout": {"tile": plan.tile, "paired": plan.paired, "lds_bytes": plan.lds_bytes,
+                          "autotune_ms": chosen["tune"]},
+        "roofline": {"kernel": f"dsx_dedisp_detect ({kname})", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(),
                      "alg_bytes_per_launch": alg_bytes, "avg_ms": dd_ms,

@@ -8,11 +8,11 @@ Stage APIs keep the names, arguments and layouts of dsa110-rt's Python
 stages; the work runs in hand-written HIP kernels of ``libdsx.so`` through
 the C ABI in ``include/dsx.h``.  There is no CPU fallback.
 """
-from .config import NodeConfig, c1_config, c2_config, c2_small_config
+from .config import NodeConfig, small_node_config, search_node_config, search_node_short_config
 from .detector import Candidate, CandidateFlags, boxcar_via_cumsum, multi_boxcar_argmax
 
 __all__ = [
-    "NodeConfig", "c1_config", "c2_config", "c2_small_config",
+    "NodeConfig", "small_node_config", "search_node_config", "search_node_short_config",
     "Candidate", "CandidateFlags", "boxcar_via_cumsum", "multi_boxcar_argmax",
 ]
 

@@ -193,38 +193,38 @@ class NodeConfig:
         return replace(self, **kw)
 
 
-def c1_config() -> NodeConfig:
+def small_node_config() -> NodeConfig:
     """16 ant x 64 ch (2.93 MHz, 1498.75 MHz down) x 2048 samples, 64 DMs 0-500."""
     dnu = 96 * DELTA_NU_CH_GHZ
     top = freq_GHz(0, 0)
     freqs = tuple(top - c * dnu for c in range(64))
     return NodeConfig(
-        name="c1_16ant_64ch", n_ant=16, chan_freqs_GHz=freqs, n_samp=2048, t_int=8,
+        name="16ant_64ch", n_ant=16, chan_freqs_GHz=freqs, n_samp=2048, t_int=8,
         csf=1, n_sub=64, n_grid=64, grid_freq_GHz=None,
         dm_max=500.0, n_dm=64, widths=(1, 2, 4, 8), threshold=8.0,
     )
 
 
-def c2_config() -> NodeConfig:
+def search_node_config() -> NodeConfig:
     """96 ant x 384 ch (chgroup 0) x 16384 samples; 16 subbands, 256^2 grid, 256 DMs."""
     freqs = tuple(freq_GHz(0, c) for c in range(384))
     return NodeConfig(
-        name="c2_96ant_384ch", n_ant=96, chan_freqs_GHz=freqs, n_samp=16384, t_int=8,
+        name="96ant_384ch", n_ant=96, chan_freqs_GHz=freqs, n_samp=16384, t_int=8,
         csf=24, n_sub=16, n_grid=256, grid_freq_GHz=1.4,
         dm_max=1500.0, n_dm=256, widths=(1, 2, 4, 8, 16, 32), threshold=8.0,
         layout_seed=0xD5A11096, volt_seed=0xD5A11097,
     )
 
 
-def c4_config() -> NodeConfig:
-    """Config-2 inputs with the uneven DM plan: 0-300 (128 trials, full time
+def uneven_dm_config() -> NodeConfig:
+    """96-antenna 384-channel inputs with the uneven DM plan: 0-300 (128 trials, full time
     resolution), 300-800 (96 trials, 2x downsampled), 800-1500 (64 trials, 4x
     downsampled); boxcars 1-64 samples of each range's resolution."""
-    return c2_config().with_(
-        name="c4_96ant_384ch_uneven", widths=(1, 2, 4, 8, 16, 32, 64), n_dm=288, dm_max=1500.0,
+    return search_node_config().with_(
+        name="96ant_384ch_uneven", widths=(1, 2, 4, 8, 16, 32, 64), n_dm=288, dm_max=1500.0,
         dm_ranges=((0.0, 300.0, 128, 1, False), (300.0, 800.0, 96, 2, False), (800.0, 1500.0, 64, 4, True)))
 
 
-def c2_small_config(n_samp: int = 1024) -> NodeConfig:
+def search_node_short_config(n_samp: int = 1024) -> NodeConfig:
     """The 96-antenna node at a short block, for parity tests."""
-    return c2_config().with_(name=f"c2_96ant_384ch_{n_samp}", n_samp=n_samp)
+    return search_node_config().with_(name=f"96ant_384ch_{n_samp}", n_samp=n_samp)

@@ -232,15 +232,7 @@ class SearchNode:
             self.blk_count.copy_(self._blk_pinned[slot], non_blocking=True)
             n_cand.zero_()
             self._mark("dedisp", 0)
-            _lib.call("dsx_dedisp_detect", _lib.ptr(self.rings[self.cur]),
-                      _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
-                      cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(pd["sub_info"]),
-                      _lib.ptr(pd["rows"]) if plan.n_rows else None, plan.n_rows, _lib.ptr(pd["tbase"]),
-                      plan.n_terms, cfg.n_dm, _lib.ptr(self.valid_from), _lib.ptr(self.n_prev),
-                      _lib.ptr(self.blk_count), _lib.ptr(self.mean), _lib.ptr(self.m2), c_lo, n_steps,
-                      self.n_widths, plan.tile, plan.r_total, plan.q_total, float(cfg.threshold),
-                      float(cfg.min_stat_count), _lib.ptr(cand), self.cand_cap, _lib.ptr(n_cand),
-                      _lib.stream(s))
+            self._launch_dedisp(plan, pd, self.mean, self.m2, cand, n_cand, T0, n_steps, s)
             self._mark("dedisp", 1)
             self.n_prev += self.blk_count
             self._cnt_pinned[slot].copy_(n_cand, non_blocking=True)
@@ -250,6 +242,63 @@ class SearchNode:
         self.n_prev_host += counts
         self.T0 = T0 + n_steps
         return PendingBlock(slot=slot, T0=T0, n_steps=n_steps, done=done, stream=s)
+
+    def _launch_dedisp(self, plan, pd, mean, m2, cand, n_cand, T0, n_steps, s) -> None:
+        cfg = self.cfg
+        _lib.call("dsx_dedisp_detect", _lib.ptr(self.rings[self.cur]),
+                  _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
+                  cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(pd["sub_info"]),
+                  _lib.ptr(pd["rows"]) if plan.n_rows else None, plan.n_rows, _lib.ptr(pd["tbase"]),
+                  plan.n_terms, cfg.n_dm, _lib.ptr(self.valid_from), _lib.ptr(self.n_prev),
+                  _lib.ptr(self.blk_count), _lib.ptr(mean), _lib.ptr(m2), T0 - DETECT_DELAY_STEPS, n_steps,
+                  self.n_widths, plan.tile, plan.r_total, plan.q_total, float(cfg.threshold),
+                  float(cfg.min_stat_count), _lib.ptr(cand), self.cand_cap, _lib.ptr(n_cand), _lib.stream(s))
+
+    def autotune(self, n_steps: int, *, budgets=(48 << 10, 64 << 10, 80 << 10, 112 << 10, 160 << 10),
+                 modes=(2, 1), reps: int = 2) -> List[dict]:
+        """Time the dedispersion kernel under each LDS layout the planner
+        offers (time tile from the LDS budget; unpaired or subband-pair rows)
+        on the current history and keep the fastest for blocks of ``n_steps``.
+        Runs on scratch copies of the statistics and candidate buffers, so the
+        stream's state is untouched.  Returns the timings."""
+        s = self._stream(None)
+        mean, m2 = self.mean.clone(), self.m2.clone()
+        cand = torch.empty_like(self.cand_slots[0])
+        n_cand = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        seen, rows, best = set(), [], None
+        kw = dict(n_widths=self.n_widths, n_centres=n_steps, ring_dtype=self.ring_dtype)
+        for mode in modes:
+            for b in budgets:
+                try:
+                    plan = plan_dedisp(self.shifts_host.T, lds_budget=b, mode=mode, **kw)
+                except RuntimeError:
+                    continue
+                key = (plan.tile, plan.paired)
+                if key in seen:
+                    continue
+                seen.add(key)
+                pd = {k: torch.from_numpy(np.ascontiguousarray(getattr(plan, k))).to(self.dev)
+                      for k in ("sub_info", "rows", "tbase")}
+                ms = []
+                with torch.cuda.stream(s):
+                    for _ in range(reps):
+                        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                        mean.copy_(self.mean)
+                        m2.copy_(self.m2)
+                        n_cand.zero_()
+                        e0.record(s)
+                        self._launch_dedisp(plan, pd, mean, m2, cand, n_cand, self.T0, n_steps, s)
+                        e1.record(s)
+                        e1.synchronize()
+                        ms.append(e0.elapsed_time(e1))
+                t = min(ms)
+                rows.append({"tile": plan.tile, "paired": plan.paired, "lds_bytes": plan.lds_bytes, "ms": t})
+                if best is None or t < best[0]:
+                    best = (t, plan, pd)
+        if best is None:
+            raise RuntimeError("no dedispersion layout fits")
+        self.plans[n_steps] = (best[1], best[2])
+        return rows
 
     def finish(self, pend: "PendingBlock") -> BlockResult:
         """Wait for a block's detection and decode its candidates."""
@@ -349,6 +398,14 @@ class MultiRangeNode:
             off += int(n)
         self.widths_all = [w * f for _, f, _, _ in self.ranges for w in cfg.widths]
         self.ring_dtype = ring_dtype
+
+    def autotune(self, n_samp: Optional[int] = None) -> List[List[dict]]:
+        """Pick each range's dedispersion LDS layout for blocks of ``n_samp``
+        samples by timing (:meth:`SearchNode.autotune`); returns the timings
+        per range."""
+        n_samp = self.cfg.n_samp if n_samp is None else int(n_samp)
+        n_steps = self.front_node._check_block(n_samp)
+        return [nd.autotune(n_steps // f) for nd, f, _, _ in self.ranges]
 
     def process_block(self, raw: torch.Tensor, n_samp: Optional[int] = None) -> BlockResult:
         cfg = self.cfg

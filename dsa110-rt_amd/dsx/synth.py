@@ -288,7 +288,7 @@ def make_scenario(cfg: NodeConfig, bursts: Sequence[Burst] = ()) -> Scenario:
     return Scenario(cfg=cfg, antpos_e=e, antpos_n=n, cmap=cmap, bursts=list(cal), tables=tables)
 
 
-def c1_bursts(cfg: NodeConfig) -> List[Burst]:
+def small_node_bursts(cfg: NodeConfig) -> List[Burst]:
     """Config 1: one burst, DM 250, width 2 steps, S/N 20, seeded pixel, in block 5.
 
     Block 5 is the first block whose DM-250 statistics hold 512 complete

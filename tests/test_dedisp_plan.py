@@ -49,15 +49,15 @@ def _model(plan, sh_sd, n_widths, tt, rng):
     return D, want
 
 
-@pytest.mark.parametrize("name,mode,dtype", [("c2", 0, torch.float16), ("c2", 2, torch.float16),
-                                              ("c2", 1, torch.float32), ("c1", 0, torch.float16)])
+@pytest.mark.parametrize("name,mode,dtype", [("96ant", 0, torch.float16), ("96ant", 2, torch.float16),
+                                              ("96ant", 1, torch.float32), ("16ant", 0, torch.float16)])
 def test_plan_reproduces_direct_dedispersion(name, mode, dtype):
-    cfg = dcfg.c2_config() if name == "c2" else dcfg.c1_config()
+    cfg = dcfg.search_node_config() if name == "96ant" else dcfg.small_node_config()
     sh_sd = np.ascontiguousarray(cfg.shift_table().T)
     nw = len(cfg.widths)
     plan = plan_dedisp(sh_sd, n_widths=nw, n_centres=cfg.n_steps, ring_dtype=dtype,
-                       lds_budget=80 * 1024 if name == "c2" else 160 * 1024, mode=mode)
-    assert plan.lds_bytes <= (80 if name == "c2" else 160) * 1024
+                       lds_budget=80 * 1024 if name == "96ant" else 160 * 1024, mode=mode)
+    assert plan.lds_bytes <= (80 if name == "96ant" else 160) * 1024
     assert cfg.n_steps % plan.tile == 0 and plan.tile % 64 == 0
     if mode == 1:
         assert plan.paired
@@ -67,8 +67,8 @@ def test_plan_reproduces_direct_dedispersion(name, mode, dtype):
     np.testing.assert_allclose(D, want, rtol=1e-12, atol=1e-12)
 
 
-def test_plan_c2_is_paired_with_12_deltas():
-    cfg = dcfg.c2_config()
+def test_plan_96ant_is_paired_with_12_deltas():
+    cfg = dcfg.search_node_config()
     plan = plan_dedisp(cfg.shift_table().T, n_widths=6, n_centres=2048, ring_dtype=torch.float16,
                        lds_budget=80 * 1024)
     assert plan.paired and plan.n_terms == 8 and plan.n_rows == 96 and plan.lds_bytes <= 80 * 1024

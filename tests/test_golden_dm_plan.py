@@ -108,7 +108,7 @@ def test_delay_native_samples_table():
 
 
 def test_node_shift_table_matches_oracle():
-    cfg = dcfg.c2_config()
+    cfg = dcfg.search_node_config()
     assert np.array_equal(node_shift_table(cfg.dm_trials, cfg.subband_ref_GHz, cfg.nu_bot_GHz, cfg.t_step_us),
                           osearch.shift_table(cfg.dm_trials, cfg.subband_ref_GHz, cfg.nu_bot_GHz, cfg.t_step_us))
     assert np.array_equal(node_shift_table(cfg.dm_trials, cfg.subband_ref_GHz, cfg.nu_bot_GHz, cfg.t_step_us),

@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def c1():
-    cfg = dcfg.c1_config().with_(n_samp=256)
+    cfg = dcfg.small_node_config().with_(n_samp=256)
     sc = synth.make_scenario(cfg)
     return cfg, sc, sc.block(0, 256)
 

@@ -1,6 +1,6 @@
 """The plain FP32 form (library GEMM correlator, index_add_ gridder, rocFFT
 imager, one dedispersion pass per DM trial) against the float64 oracle and
-the fused form on the config-1 stream (16 ant x 64 ch, 7 blocks, DM-250
+the fused form on the 16-antenna stream (16 ant x 64 ch, 7 blocks, DM-250
 burst).
 
 Bars: visibilities bit-exact (integer-valued complex64); decoded candidate
@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_plain_correlator_exact():
-    cfg = dcfg.c1_config().with_(n_samp=512)
+    cfg = dcfg.small_node_config().with_(n_samp=512)
     sc = synth.make_scenario(cfg)
     raw = sc.block(0, 256)
     node = PlainNode(cfg, sc.antpos_e, sc.antpos_n)
@@ -32,8 +32,8 @@ def test_plain_correlator_exact():
 
 
 def test_plain_c1_stream_candidates_match_oracle():
-    cfg = dcfg.c1_config()
-    sc = synth.make_scenario(cfg, synth.c1_bursts(cfg))
+    cfg = dcfg.small_node_config()
+    sc = synth.make_scenario(cfg, synth.small_node_bursts(cfg))
     gs = synth.GpuSynth(sc)
     node = PlainNode(cfg, sc.antpos_e, sc.antpos_n)
     o = OracleStream(cfg, sc.antpos_e, sc.antpos_n)

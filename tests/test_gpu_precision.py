@@ -25,8 +25,8 @@ pytestmark = pytest.mark.gpu
 
 
 def test_precision_report_c1():
-    cfg = dcfg.c1_config()
-    sc = synth.make_scenario(cfg, synth.c1_bursts(cfg))
+    cfg = dcfg.small_node_config()
+    sc = synth.make_scenario(cfg, synth.small_node_bursts(cfg))
     n_blk = 3
     raw = np.concatenate([sc.block(b * cfg.n_samp, cfg.n_samp) for b in range(n_blk)])
     n_samp = n_blk * cfg.n_samp
@@ -59,7 +59,7 @@ def test_precision_report_c1():
                              time_shift_per_chgroup=sh[d].astype(np.int32), t_window=(t_lo, t_hi),
                              n_chgroup=cfg.n_sub).cpu().numpy().reshape(t_hi - t_lo, -1)
         ser_err[d] = np.abs(D - ref_D[d]).max(axis=0) / np.maximum(np.abs(ref_D[d]).max(axis=0), 1e-30)
-    report = {"config": "config-1 inputs, 3 blocks (768 steps), 64 subbands, 64 DMs, crop 32x32",
+    report = {"config": "16-antenna 64-channel inputs, 3 blocks (768 steps), 64 subbands, 64 DMs, crop 32x32",
               "visibilities_max_abs_err": vis_err,
               "image_fp16_rel_err": {"max": float(img_err.max()), "median": float(np.median(img_err)),
                                      "n_images": int(img_err.size)},

@@ -1,5 +1,5 @@
 """GPU dedispersion + detection parity against the float64 oracle, and the
-whole config-1 stream (16 ant x 64 ch, 7 blocks, DM-250 burst) end to end.
+whole 16-antenna stream (16 ant x 64 ch, 7 blocks, DM-250 burst) end to end.
 
 Bars: chunk-maximum records identical in (DM, chunk, pixel, time, width)
 with S/N within 1e-4 relative for float32 images; decoded candidate keys
@@ -44,13 +44,17 @@ def _cand_keys(cands):
     return [(c.dm_idx, c.time_bin, c.width_samples, int(c.l), int(c.m)) for c in cands]
 
 
-def test_dedisp_detect_random_images_vs_oracle():
-    cfg = dcfg.c1_config().with_(n_sub=8, n_grid=16, dm_max=100.0, n_dm=64, threshold=4.5,
+@pytest.mark.parametrize("layout", ["unpaired", "paired", "autotuned"])
+def test_dedisp_detect_random_images_vs_oracle(layout):
+    """The LDS layouts the planner offers (unpaired subband windows, subband
+    pair rows, and whichever the autotuner keeps) give the same records."""
+    cfg = dcfg.small_node_config().with_(n_sub=8, n_grid=16, dm_max=100.0, n_dm=64, threshold=4.5,
                                  min_stat_count=256, widths=(1, 2, 4, 8, 16))
     rng = np.random.default_rng(123)
     ant_e = rng.uniform(-500, 500, cfg.n_ant)
     ant_n = rng.uniform(-100, 100, cfg.n_ant)
-    node = SearchNode(cfg, ant_e, ant_n, ring_dtype=torch.float32)
+    print(f"layout {layout}", flush=True)
+    node = SearchNode(cfg, ant_e, ant_n, ring_dtype=torch.float32, dedisp_mode=1 if layout == "paired" else 2)
     o = OracleStream(cfg, ant_e, ant_n)
     sh = cfg.shift_table()
     L = cfg.n_steps
@@ -66,6 +70,13 @@ def test_dedisp_detect_random_images_vs_oracle():
                     img[s, tt:tt + 4, 77] += 2.5
         img_t = torch.from_numpy(img.reshape(cfg.n_sub, L, N, N)).cuda()
         node.push_images(img_t, L)
+        if layout == "autotuned" and blk == 1:
+            rows = node.autotune(L)
+            assert len(rows) >= 2 and {r["paired"] for r in rows} == {False, True}
+            best = min(rows, key=lambda r: r["ms"])
+            assert node.plans[L][0].tile == best["tile"] and node.plans[L][0].paired == best["paired"]
+        if layout == "paired":
+            assert node._plan_for(L)[0].paired
         res = node.detect(L)
         o.push_images(img.astype(np.float64))
         g = _gpu_records(res.records)
@@ -80,9 +91,9 @@ def test_dedisp_detect_random_images_vs_oracle():
 
 
 @pytest.mark.parametrize("ring_dtype", [torch.float16, torch.float32])
-def test_c1_stream_candidates_match_oracle(ring_dtype):
-    cfg = dcfg.c1_config()
-    sc = synth.make_scenario(cfg, synth.c1_bursts(cfg))
+def test_16ant_stream_candidates_match_oracle(ring_dtype):
+    cfg = dcfg.small_node_config()
+    sc = synth.make_scenario(cfg, synth.small_node_bursts(cfg))
     gs = synth.GpuSynth(sc)
     node = SearchNode(cfg, sc.antpos_e, sc.antpos_n, ring_dtype=ring_dtype)
     o = OracleStream(cfg, sc.antpos_e, sc.antpos_n)

@@ -20,7 +20,7 @@ LENGTHS = (1024, 2048, 4096)
 
 
 def _stream(seed_volt, seed_len, n_blocks, bursts):
-    cfg = dcfg.c1_config().with_(volt_seed=seed_volt)
+    cfg = dcfg.small_node_config().with_(volt_seed=seed_volt)
     sc = synth.make_scenario(cfg, bursts(cfg) if bursts else ())
     rng = np.random.default_rng(seed_len)
     lens = [int(x) for x in rng.choice(LENGTHS, size=n_blocks)]
@@ -38,7 +38,7 @@ def _keys(cands):
 
 def test_alone_equals_in_the_mix():
     cap = max(LENGTHS) // 8
-    streams = [_stream(0xD5A1107F, 11, 10, synth.c1_bursts), _stream(0x5EED0002, 12, 8, None)]
+    streams = [_stream(0xD5A1107F, 11, 10, synth.small_node_bursts), _stream(0x5EED0002, 12, 8, None)]
     # alone: each stream on its own node, one block at a time
     alone = []
     for cfg, sc, lens, starts in streams:

@@ -18,13 +18,13 @@ from oracle import image as oimage
 pytestmark = pytest.mark.gpu
 
 
-def _scenario_c1():
-    cfg = dcfg.c1_config()
-    return synth.make_scenario(cfg, synth.c1_bursts(cfg))
+def _scenario_small():
+    cfg = dcfg.small_node_config()
+    return synth.make_scenario(cfg, synth.small_node_bursts(cfg))
 
 
 def test_synth_gpu_matches_numpy_twin():
-    sc = _scenario_c1()
+    sc = _scenario_small()
     cfg = sc.cfg
     gs = synth.GpuSynth(sc)
     for blk in (0, 5):          # block 5 carries the burst
@@ -35,8 +35,8 @@ def test_synth_gpu_matches_numpy_twin():
         assert np.array_equal(got, want), f"block {blk}: {np.count_nonzero(got != want)} bytes differ"
 
 
-def test_synth_gpu_c2_slice_matches_numpy():
-    cfg = dcfg.c2_small_config(64)
+def test_synth_gpu_96ant_slice_matches_numpy():
+    cfg = dcfg.search_node_short_config(64)
     sc = synth.make_scenario(cfg, synth.random_bursts(cfg, 2, 7, t_lo_step=0, t_hi_step=8))
     gs = synth.GpuSynth(sc)
     got = gs.block(128, 64).cpu().numpy()
@@ -68,8 +68,8 @@ def _corr_gpu(raw_np, cfg, *, fuse_pol=True, out_layout=1, n_samp=None, split=Fa
     return vis.cpu().numpy()
 
 
-def test_corr_c1_bit_exact():
-    sc = _scenario_c1()
+def test_corr_16ant_bit_exact():
+    sc = _scenario_small()
     cfg = sc.cfg
     raw = sc.block(5 * cfg.n_samp, cfg.n_samp)
     got = _corr_gpu(raw, cfg)[:, :, 0]                       # (steps, cout, nbase, 2)
@@ -81,7 +81,7 @@ def test_corr_c1_bit_exact():
 @pytest.mark.parametrize("fuse_pol,out_layout,split", [(False, 1, False), (True, 0, False),
                                                         (False, 0, True), (True, 1, True)])
 def test_corr_layouts_bit_exact(fuse_pol, out_layout, split):
-    cfg = dcfg.c1_config().with_(csf=4, n_sub=16, n_samp=512)
+    cfg = dcfg.small_node_config().with_(csf=4, n_sub=16, n_samp=512)
     sc = synth.make_scenario(cfg)
     raw = sc.block(0, 512)
     got = _corr_gpu(raw, cfg, fuse_pol=fuse_pol, out_layout=out_layout, split=split)
@@ -94,9 +94,9 @@ def test_corr_layouts_bit_exact(fuse_pol, out_layout, split):
     assert np.array_equal(got.astype(np.int64), want)
 
 
-def test_corr_c2_shape_bit_exact():
+def test_corr_96ant_shape_bit_exact():
     """96 antennas x 384 channels, csf 24 (16 subbands), 8 steps."""
-    cfg = dcfg.c2_small_config(64)
+    cfg = dcfg.search_node_short_config(64)
     sc = synth.make_scenario(cfg)
     raw = sc.block(0, 64)
     got = _corr_gpu(raw, cfg)[:, :, 0]
@@ -106,7 +106,7 @@ def test_corr_c2_shape_bit_exact():
 
 def test_corr_sign_known_answer():
     """E_1 = (1, 0), E_2 = (0, 1) -> V = conj(E_1) E_2 = +i per sample and pol."""
-    cfg = dcfg.c1_config().with_(n_samp=64)
+    cfg = dcfg.small_node_config().with_(n_samp=64)
     re = np.zeros((64, cfg.n_ant, cfg.n_chan, 2), dtype=np.int64)
     im = np.zeros_like(re)
     re[:, 3] = 1
@@ -120,8 +120,8 @@ def test_corr_sign_known_answer():
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-5), (torch.float16, 1e-3)])
-def test_imager_c1_vs_float64(dtype, tol):
-    sc = _scenario_c1()
+def test_imager_16ant_vs_float64(dtype, tol):
+    sc = _scenario_small()
     cfg = sc.cfg
     raw = sc.block(5 * cfg.n_samp, cfg.n_samp)[: 64]          # 16 steps
     vis = ocorr.correlate(raw, n_pkt=64, n_ant=cfg.n_ant, n_chan=cfg.n_chan, t_int=cfg.t_int, csf=cfg.csf)
@@ -133,8 +133,8 @@ def test_imager_c1_vs_float64(dtype, tol):
     assert err <= tol, f"max err {err:.3e} > {tol}"
 
 
-def test_imager_c2_vs_float64():
-    cfg = dcfg.c2_small_config(64)
+def test_imager_96ant_vs_float64():
+    cfg = dcfg.search_node_short_config(64)
     sc = synth.make_scenario(cfg)
     raw = sc.block(0, 64)
     vis = ocorr.correlate(raw, n_pkt=32, n_ant=96, n_chan=384, t_int=8, csf=24)

@@ -1,5 +1,5 @@
 """Uneven DM plan (three DM ranges at 1x, 2x and 4x time resolution, widths
-1-64 samples of each range's resolution) on config-1 inputs against the
+1-64 samples of each range's resolution) on 16-antenna 64-channel inputs against the
 float64 oracle: per block, the chunk-maximum records of every range,
 converted to full-resolution steps and global DM indices, are identical in
 (DM, time, pixel, width) with S/N within 2e-3 relative (the GPU history of a
@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _cfg():
-    return dcfg.c1_config().with_(
+    return dcfg.small_node_config().with_(
         widths=(1, 2, 4, 8, 16, 32, 64), threshold=6.0, min_stat_count=128, n_dm=72,
         dm_ranges=((0.0, 150.0, 32, 1, False), (150.0, 300.0, 24, 2, False), (300.0, 500.0, 16, 4, True)))
 
@@ -36,7 +36,7 @@ def _oracle_records(streams, factors, offsets, n_w, thr):
 def test_uneven_plan_records_match_oracle():
     cfg = _cfg()
     cfg.validate()
-    sc = synth.make_scenario(cfg, synth.c1_bursts(cfg))
+    sc = synth.make_scenario(cfg, synth.small_node_bursts(cfg))
     gs = synth.GpuSynth(sc)
     node = MultiRangeNode(cfg, sc.antpos_e, sc.antpos_n)
     base = OracleStream(cfg.with_(dm_ranges=()), sc.antpos_e, sc.antpos_n)

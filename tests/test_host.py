@@ -14,7 +14,7 @@ from oracle import image as oimage
 from oracle import search as osearch
 
 
-@pytest.mark.parametrize("make", [dcfg.c1_config, dcfg.c2_config])
+@pytest.mark.parametrize("make", [dcfg.small_node_config, dcfg.search_node_config])
 def test_cell_map_matches_oracle(make):
     cfg = make()
     e, n = synth.antenna_layout(cfg)
@@ -24,7 +24,7 @@ def test_cell_map_matches_oracle(make):
     assert np.array_equal(cm.row, rows) and np.array_equal(cm.col, cols)
 
 
-@pytest.mark.parametrize("make", [dcfg.c1_config, dcfg.c2_config])
+@pytest.mark.parametrize("make", [dcfg.small_node_config, dcfg.search_node_config])
 def test_shift_table_matches_oracle(make):
     cfg = make()
     want = osearch.shift_table(cfg.dm_trials, cfg.subband_ref_GHz, cfg.nu_bot_GHz, cfg.t_step_us)
@@ -54,7 +54,7 @@ def _fold_image_numpy(vis_slab, taps, s, n_grid):
     return out
 
 
-@pytest.mark.parametrize("make", [dcfg.c1_config, dcfg.c2_config])
+@pytest.mark.parametrize("make", [dcfg.small_node_config, dcfg.search_node_config])
 def test_imager_fold_matches_reference_image(make):
     """The tap table + fold + C2R equals mask * fftshift(Re(ifft2(G)))."""
     cfg = make()
@@ -101,16 +101,16 @@ def test_noise_lut_and_quantised_variance():
 
 
 def test_synth_stream_independent_of_block_split():
-    cfg = dcfg.c1_config().with_(n_samp=256)
+    cfg = dcfg.small_node_config().with_(n_samp=256)
     sc = synth.make_scenario(cfg)
     whole = sc.block(0, 256)
     parts = np.concatenate([sc.block(0, 64), sc.block(64, 192)])
     assert np.array_equal(whole, parts)
 
 
-def test_c1_burst_calibration_targets_snr():
-    cfg = dcfg.c1_config()
-    sc = synth.make_scenario(cfg, synth.c1_bursts(cfg))
+def test_small_node_burst_calibration_targets_snr():
+    cfg = dcfg.small_node_config()
+    sc = synth.make_scenario(cfg, synth.small_node_bursts(cfg))
     b = sc.bursts[0]
     assert b.amp > 0
     unit = synth.expected_unit_snr(cfg, sc.cmap, sc.antpos_e, sc.antpos_n,
@@ -128,7 +128,7 @@ def test_decode_chunk_maxima_matches_oracle_decode():
     rec["d"] = rng.integers(0, 64, n)
     rec["k"] = rng.integers(0, 4, n)
     rec["snr"] = rng.uniform(8, 30, n).astype(np.float32)
-    cfg = dcfg.c1_config()
+    cfg = dcfg.small_node_config()
     got = decode_chunk_maxima(rec, widths=cfg.widths, n_grid=64, dm_trials=cfg.dm_trials,
                               threshold=8.0, n_top=100, radius_t=128, radius_dm=5, radius_lm=3,
                               t_int=8)
@@ -148,13 +148,13 @@ def test_decode_chunk_maxima_matches_oracle_decode():
 
 def test_config_validation():
     with pytest.raises(ValueError):
-        dcfg.c1_config().with_(widths=(1, 3)).validate()
+        dcfg.small_node_config().with_(widths=(1, 3)).validate()
     with pytest.raises(ValueError):
-        dcfg.c1_config().with_(n_sub=7).validate()
+        dcfg.small_node_config().with_(n_sub=7).validate()
     with pytest.raises(ValueError):
-        dcfg.c1_config().with_(n_samp=1000).validate()
-    dcfg.c1_config().validate()
-    dcfg.c2_config().validate()
+        dcfg.small_node_config().with_(n_samp=1000).validate()
+    dcfg.small_node_config().validate()
+    dcfg.search_node_config().validate()
 
 
 def test_dispersion_sign_and_zero():
