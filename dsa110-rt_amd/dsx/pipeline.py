@@ -205,6 +205,10 @@ class SearchNode:
         if n_steps > self.cap:
             raise ValueError(f"{n_steps} steps > max_block_steps {self.cap}")
         _lib.require_gpu(img)
+        if img.dtype != self.ring_dtype:
+            raise TypeError(f"images are {img.dtype}; this history holds {self.ring_dtype}")
+        if img.numel() < cfg.n_sub * n_steps * self.n_pix or not img.is_contiguous():
+            raise ValueError(f"images must be contiguous [n_sub, >= {n_steps}, n_grid, n_grid]")
         old, new = self.rings[self.cur], self.rings[1 - self.cur]
         self._mark("ring", 0)
         _lib.call("dsx_ring_advance", _lib.ptr(old), _lib.ptr(new), _lib.ptr(img), cfg.n_sub,
@@ -254,7 +258,7 @@ class SearchNode:
                   self.n_widths, plan.tile, plan.r_total, plan.q_total, float(cfg.threshold),
                   float(cfg.min_stat_count), _lib.ptr(cand), self.cand_cap, _lib.ptr(n_cand), _lib.stream(s))
 
-    def autotune(self, n_steps: int, *, budgets=(48 << 10, 64 << 10, 80 << 10, 112 << 10, 160 << 10),
+    def autotune(self, n_steps: int, *, budgets=(16 << 10, 24 << 10, 32 << 10, 48 << 10, 64 << 10, 80 << 10, 112 << 10, 160 << 10),
                  modes=(2, 1), reps: int = 2) -> List[dict]:
         """Time the dedispersion kernel under each LDS layout the planner
         offers (time tile from the LDS budget; unpaired or subband-pair rows)

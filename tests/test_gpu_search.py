@@ -44,17 +44,21 @@ def _cand_keys(cands):
     return [(c.dm_idx, c.time_bin, c.width_samples, int(c.l), int(c.m)) for c in cands]
 
 
-@pytest.mark.parametrize("layout", ["unpaired", "paired", "autotuned"])
+@pytest.mark.parametrize("layout", ["unpaired", "paired", "autotuned", "fp16", "fp16-autotuned"])
 def test_dedisp_detect_random_images_vs_oracle(layout):
     """The LDS layouts the planner offers (unpaired subband windows, subband
-    pair rows, and whichever the autotuner keeps) give the same records."""
+    pair rows, and whichever the autotuner keeps) give the oracle's records,
+    for float32 and fp16 histories (fp16 cases feed images already rounded
+    to fp16, so both sides sum the same values)."""
     cfg = dcfg.small_node_config().with_(n_sub=8, n_grid=16, dm_max=100.0, n_dm=64, threshold=4.5,
-                                 min_stat_count=256, widths=(1, 2, 4, 8, 16))
+                                         min_stat_count=256, widths=(1, 2, 4, 8, 16))
     rng = np.random.default_rng(123)
     ant_e = rng.uniform(-500, 500, cfg.n_ant)
     ant_n = rng.uniform(-100, 100, cfg.n_ant)
     print(f"layout {layout}", flush=True)
-    node = SearchNode(cfg, ant_e, ant_n, ring_dtype=torch.float32, dedisp_mode=1 if layout == "paired" else 2)
+    fp16 = layout.startswith("fp16")
+    dt = torch.float16 if fp16 else torch.float32
+    node = SearchNode(cfg, ant_e, ant_n, ring_dtype=dt, dedisp_mode=1 if layout == "paired" else 2)
     o = OracleStream(cfg, ant_e, ant_n)
     sh = cfg.shift_table()
     L = cfg.n_steps
@@ -68,9 +72,11 @@ def test_dedisp_detect_random_images_vs_oracle(layout):
                 tt = t0 - blk * L - int(sh[40, s])
                 if 0 <= tt < L - 4:
                     img[s, tt:tt + 4, 77] += 2.5
-        img_t = torch.from_numpy(img.reshape(cfg.n_sub, L, N, N)).cuda()
+        if fp16:
+            img = img.astype(np.float16).astype(np.float32)
+        img_t = torch.from_numpy(img.reshape(cfg.n_sub, L, N, N)).cuda().to(dt)
         node.push_images(img_t, L)
-        if layout == "autotuned" and blk == 1:
+        if layout.endswith("autotuned") and blk == 1:
             rows = node.autotune(L)
             assert len(rows) >= 2 and {r["paired"] for r in rows} == {False, True}
             best = min(rows, key=lambda r: r["ms"])
