@@ -19,6 +19,7 @@ namespace dsx {
 
 constexpr int IMG_THREADS = 256;
 constexpr int IMG_MAX_KB = 32;          // <= 64 rows per chunk (2 rows = 4 K values per MFMA)
+constexpr int IMG_TAP_BATCH = 8;        // taps per thread with their loads in flight together
 constexpr int TAP_CONJ = 1 << 20;
 constexpr int TAP_NEG = 1 << 21;
 constexpr int TAP_CELL = (1 << 20) - 1;
@@ -86,7 +87,7 @@ struct ImgParams {
 constexpr int R16_RS = 272;
 
 template <typename TOut, bool R16>
-__global__ __launch_bounds__(IMG_THREADS) void image_kernel(ImgParams P) {
+__global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
     extern __shared__ __align__(16) float2 smem[];
     const int N = P.N;
     const int RS = R16 ? R16_RS : N;         // row stride of H
@@ -142,19 +143,38 @@ __global__ __launch_bounds__(IMG_THREADS) void image_kernel(ImgParams P) {
             } else {
             for (int i = tid; i < rc * RS; i += IMG_THREADS) Hi[i] = make_int2(0, 0);
             __syncthreads();
-            // ---- 1. scatter taps (int32 LDS atomics)
-            for (int i = tap_lo + tid; i < tap_hi; i += IMG_THREADS) {
-                const int d = P.tap_dst[i];
-                const int cell = d & TAP_CELL;
-                const int r = (cell >> P.log2N) - r0;
-                if (r < 0 || r >= rc) continue;
-                const int w = cell & (N - 1);
-                int2 v = *reinterpret_cast<const int2 *>(vslab + 2 * (int64_t)P.tap_src[i]);
-                if (d & TAP_CONJ) v.y = -v.y;
-                if (d & TAP_NEG) { v.x = -v.x; v.y = -v.y; }
-                int *c = reinterpret_cast<int *>(&Hi[r * RS + w]);
-                atomicAdd(c, v.x);
-                atomicAdd(c + 1, v.y);
+            // ---- 1. scatter taps (int32 LDS atomics).  A thread takes its taps in
+            //      batches of IMG_TAP_BATCH with every load of a batch in flight
+            //      together (tap tables, then the visibilities they name); loads use
+            //      a valid index for every slot, slots past the end add nothing.
+            for (int i0 = tap_lo + tid; i0 < tap_hi; i0 += IMG_THREADS * IMG_TAP_BATCH) {
+                int dst[IMG_TAP_BATCH], src[IMG_TAP_BATCH];
+#pragma unroll
+                for (int k = 0; k < IMG_TAP_BATCH; ++k) {
+                    const int i = i0 + k * IMG_THREADS;
+                    const int ic = i < tap_hi ? i : tap_lo;
+                    dst[k] = P.tap_dst[ic];
+                    src[k] = P.tap_src[ic];
+                }
+                int2 v[IMG_TAP_BATCH];
+#pragma unroll
+                for (int k = 0; k < IMG_TAP_BATCH; ++k)
+                    v[k] = *reinterpret_cast<const int2 *>(vslab + 2 * (int64_t)src[k]);
+#pragma unroll
+                for (int k = 0; k < IMG_TAP_BATCH; ++k) {
+                    const int d = dst[k];
+                    const int cell = d & TAP_CELL;
+                    const int r = (cell >> P.log2N) - r0;
+                    if (i0 + k * IMG_THREADS < tap_hi && r >= 0 && r < rc) {
+                        const int w = cell & (N - 1);
+                        int2 x = v[k];
+                        if (d & TAP_CONJ) x.y = -x.y;
+                        if (d & TAP_NEG) { x.x = -x.x; x.y = -x.y; }
+                        int *c = reinterpret_cast<int *>(&Hi[r * RS + w]);
+                        atomicAdd(c, x.x);
+                        atomicAdd(c + 1, x.y);
+                    }
+                }
             }
             __syncthreads();
             // ---- int32 -> float32 in place, with the bit-reversal permutation

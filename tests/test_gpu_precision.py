@@ -24,7 +24,7 @@ from oracle import search as osearch
 pytestmark = pytest.mark.gpu
 
 
-def test_precision_report_c1():
+def test_precision_report_16ant():
     cfg = dcfg.small_node_config()
     sc = synth.make_scenario(cfg, synth.small_node_bursts(cfg))
     n_blk = 3
