@@ -19,7 +19,7 @@ namespace dsx {
 
 constexpr int IMG_THREADS = 256;
 constexpr int IMG_MAX_KB = 32;          // <= 64 rows per chunk (2 rows = 4 K values per MFMA)
-constexpr int IMG_TAP_BATCH = 8;        // taps per thread with their loads in flight together
+constexpr int IMG_TAP_BATCH = 24;        // taps per thread with their loads in flight together
 constexpr int TAP_CONJ = 1 << 20;
 constexpr int TAP_NEG = 1 << 21;
 constexpr int TAP_CELL = (1 << 20) - 1;

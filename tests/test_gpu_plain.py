@@ -31,7 +31,7 @@ def test_plain_correlator_exact():
     assert np.array_equal(v.imag.astype(np.int64), want[..., 1])
 
 
-def test_plain_c1_stream_candidates_match_oracle():
+def test_plain_16ant_stream_candidates_match_oracle():
     cfg = dcfg.small_node_config()
     sc = synth.make_scenario(cfg, synth.small_node_bursts(cfg))
     gs = synth.GpuSynth(sc)
