@@ -96,6 +96,45 @@ def test_dedisp_detect_random_images_vs_oracle(layout):
     assert total > 0
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_dedisp_detect_ragged_shapes_vs_oracle(mode):
+    """Ragged shapes: 300 DM trials (two work-groups along DMs, the second
+    with 44 live lanes), 7 subbands (the last subband pair has no second
+    member), shifts up to 208 steps against 256-step blocks; pair rows
+    (mode 1) and unpaired windows (mode 2) against the float64 oracle."""
+    base = dcfg.small_node_config()
+    cfg = base.with_(chan_freqs_GHz=base.chan_freqs_GHz[:63], n_sub=7, n_grid=16, dm_max=100.0, n_dm=300,
+                     threshold=4.5, min_stat_count=256, widths=(1, 2, 4, 8, 16))
+    rng = np.random.default_rng(77)
+    ant_e = rng.uniform(-500, 500, cfg.n_ant)
+    ant_n = rng.uniform(-100, 100, cfg.n_ant)
+    print(f"mode {mode}", flush=True)
+    node = SearchNode(cfg, ant_e, ant_n, ring_dtype=torch.float32, dedisp_mode=mode)
+    o = OracleStream(cfg, ant_e, ant_n)
+    sh = cfg.shift_table()
+    L, N = cfg.n_steps, cfg.n_grid
+    assert node._plan_for(L)[0].paired == (mode == 1)
+    total = 0
+    for blk in range(5):
+        img = rng.standard_normal((cfg.n_sub, L, N * N)).astype(np.float32)
+        if blk >= 3:       # dispersed pulses at DM index 290 (second DM work-group), pixel 201
+            t0 = blk * L + 220      # arrival at the band bottom; every subband's copy is in this block
+            for s in range(cfg.n_sub):
+                tt = t0 - blk * L - int(sh[290, s])
+                if 0 <= tt < L - 8:
+                    img[s, tt:tt + 8, 201] += 2.0
+        node.push_images(torch.from_numpy(img.reshape(cfg.n_sub, L, N, N)).cuda(), L)
+        res = node.detect(L)
+        o.push_images(img.astype(np.float64))
+        g = _gpu_records(res.records)
+        _compare_records(g, _oracle_records(o, cfg.threshold), cfg.threshold, 1e-4)
+        total += len(g)
+        np.testing.assert_allclose(node.mean.cpu().numpy().T, o.st_mean, rtol=1e-5, atol=1e-6)
+        np.testing.assert_allclose(node.m2.cpu().numpy().T, o.st_m2, rtol=1e-5, atol=1e-5)
+    assert any(k[0] >= 256 for k in g), "no record from the second DM work-group"
+    assert total > 0
+
+
 @pytest.mark.parametrize("ring_dtype", [torch.float16, torch.float32])
 def test_16ant_stream_candidates_match_oracle(ring_dtype):
     cfg = dcfg.small_node_config()
