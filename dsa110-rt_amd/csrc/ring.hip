@@ -46,6 +46,59 @@ __global__ __launch_bounds__(256) void ring_transpose_kernel(const T *img, T *ri
     }
 }
 
+// fp16 history with 16-byte accesses: needs n_pix and n_new multiples of 64 and
+// hist, row_len multiples of 8 (then every 8-pixel group of an image row and
+// every 8-step group of a history row is 16-byte aligned).  A 64 x 64 tile is
+// read as 8-pixel vectors, and each thread writes 8 consecutive steps of one
+// pixel, gathered from 8 LDS rows that the wave reads contiguously.
+__global__ __launch_bounds__(256) void ring_transpose16_kernel(const _Float16 *img, _Float16 *ring, int n_sub,
+                                                               int n_pix, int n_new, int hist, int row_len) {
+    __shared__ __align__(16) uint16_t tile[64][72];    // [step][pixel], 144-byte rows
+    const int tid = threadIdx.x;
+    const int tiles_p = n_pix / 64, tiles_t = n_new / 64;
+    const int64_t n_tiles = (int64_t)n_sub * tiles_p * tiles_t;
+    for (int64_t b = blockIdx.x; b < n_tiles; b += gridDim.x) {
+        const int s = (int)(b / ((int64_t)tiles_p * tiles_t));
+        const int rem = (int)(b - (int64_t)s * tiles_p * tiles_t);
+        const int tt = rem / tiles_p, tp = rem - tt * tiles_p;
+        const int t0 = tt * 64, p0 = tp * 64;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int i = tid + 256 * k;
+            const int r = i >> 3, j = i & 7;
+            const uint4 v = *reinterpret_cast<const uint4 *>(img + ((int64_t)s * n_new + t0 + r) * n_pix + p0 + 8 * j);
+            *reinterpret_cast<uint4 *>(&tile[r][8 * j]) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int i = tid + 256 * k;
+            const int q = i & 63, c = i >> 6;
+            uint32_t w[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                w[m] = (uint32_t)tile[8 * c + 2 * m][q] | ((uint32_t)tile[8 * c + 2 * m + 1][q] << 16);
+            *reinterpret_cast<uint4 *>(ring + ((int64_t)s * n_pix + p0 + q) * row_len + hist + t0 + 8 * c) =
+                make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    }
+}
+
+// History carry in 16-byte pieces (hist, row_len, prev_len multiples of 8).
+__global__ __launch_bounds__(256) void ring_hist16_kernel(const _Float16 *old_ring, _Float16 *new_ring,
+                                                          int64_t n_rows, int hist, int row_len, int prev_len) {
+    const int hv = hist >> 3;
+    const int64_t n = n_rows * hv;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / hv;
+        const int k = (int)(i - row * hv);
+        *reinterpret_cast<uint4 *>(new_ring + row * row_len + 8 * k) =
+            *reinterpret_cast<const uint4 *>(old_ring + row * row_len + prev_len + 8 * k);
+    }
+}
+
 // out[s][t][k] = sum_{j < f} in[s][f t + j][k]: f-fold time downsampling of
 // (n_planes, n_in, n_inner) planes, float32 accumulation, one thread per output
 // element (coalesced along k).
@@ -86,7 +139,17 @@ extern "C" int dsx_ring_advance(const void *ring_old, void *ring_new, const void
     const int hb = (int)std::min<int64_t>((nh + 255) / 256, 65536);
     const int64_t n_tiles = (int64_t)n_sub * ((n_pix + 63) / 64) * ((n_new + 63) / 64);
     const int tb = (int)std::min<int64_t>(n_tiles, 65536);
-    if (dtype == DSX_F16) {
+    const bool vec16 = dtype == DSX_F16 && n_pix % 64 == 0 && n_new % 64 == 0 && hist % 8 == 0 &&
+                       row_len % 8 == 0 && prev_len % 8 == 0 && ((uintptr_t)img & 15) == 0 &&
+                       ((uintptr_t)ring_old & 15) == 0 && ((uintptr_t)ring_new & 15) == 0;
+    if (vec16) {
+        const int64_t nv = n_rows * (hist / 8);
+        if (nv > 0)
+            ring_hist16_kernel<<<(int)std::min<int64_t>((nv + 255) / 256, 65536), 256, 0, s>>>(
+                (const _Float16 *)ring_old, (_Float16 *)ring_new, n_rows, hist, row_len, prev_len);
+        ring_transpose16_kernel<<<tb, 256, 0, s>>>((const _Float16 *)img, (_Float16 *)ring_new, n_sub, n_pix,
+                                                   n_new, hist, row_len);
+    } else if (dtype == DSX_F16) {
         if (nh > 0)
             ring_hist_kernel<_Float16><<<hb, 256, 0, s>>>((const _Float16 *)ring_old, (_Float16 *)ring_new,
                                                            n_rows, hist, row_len, prev_len);

@@ -82,7 +82,8 @@ class SearchNode:
         self.n_widths = len(cfg.widths)
         self.w_max = cfg.w_max
         h2 = self.w_max // 2
-        self.hist = DETECT_DELAY_STEPS + h2 + int(sh.max()) + 1
+        # rounded up to 8 steps: 16-byte aligned history rows for the fp16 copy kernels
+        self.hist = -(-(DETECT_DELAY_STEPS + h2 + int(sh.max()) + 1) // 8) * 8
         self.cap = int(max_block_steps or cfg.n_steps)
         # ---- buffers
         nb = cfg.nbase

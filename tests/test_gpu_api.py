@@ -154,3 +154,33 @@ def test_multi_boxcar_argmax():
                        for w, o, s in zip(widths, offs, sinv)])             # (K, n_out, inner)
     np.testing.assert_allclose(mx.cpu().numpy().reshape(T - maxw, -1), scores.max(axis=0), rtol=1e-6, atol=1e-6)
     assert np.array_equal(win.cpu().numpy().reshape(T - maxw, -1), scores.argmax(axis=0))
+
+
+@pytest.mark.parametrize("dtype,hist", [(torch.float16, 24), (torch.float16, 21), (torch.float32, 24)])
+def test_ring_advance_matches_numpy(dtype, hist):
+    """dsx_ring_advance over three blocks (64, 128, 64 steps): the history
+    rows hold the last `hist` steps of the previous contents followed by the
+    new block, time-major per (subband, pixel).  hist = 24 (fp16) takes the
+    16-byte path, hist = 21 the element path; both are exact copies."""
+    from dsx import _lib
+    n_sub, n_pix, cap = 3, 128, 128
+    row_len = hist + cap
+    code = _lib.DSX_F16 if dtype == torch.float16 else _lib.DSX_F32
+    rings = [torch.zeros((n_sub, n_pix, row_len), dtype=dtype, device="cuda") for _ in range(2)]
+    want = np.zeros((n_sub, n_pix, row_len))
+    rng = np.random.default_rng(5)
+    cur, prev_len = 0, 0
+    for n_new in (64, 128, 64):
+        img = rng.standard_normal((n_sub, n_new, n_pix)).astype(np.float32)
+        img_t = torch.from_numpy(img).to(dtype).cuda().contiguous()
+        img = img_t.float().cpu().numpy()
+        _lib.call("dsx_ring_advance", _lib.ptr(rings[cur]), _lib.ptr(rings[1 - cur]), _lib.ptr(img_t), n_sub,
+                  n_pix, hist, cap, prev_len, n_new, code, None)
+        new = np.zeros_like(want)
+        new[:, :, :hist] = want[:, :, prev_len:prev_len + hist]
+        new[:, :, hist:hist + n_new] = np.transpose(img, (0, 2, 1))
+        want = new
+        cur, prev_len = 1 - cur, n_new
+        torch.cuda.synchronize()
+        got = rings[cur].float().cpu().numpy()
+        assert np.array_equal(got[:, :, :hist + n_new], want[:, :, :hist + n_new]), f"block of {n_new} steps"
