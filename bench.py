@@ -314,7 +314,9 @@ def main():
                      "alg_bytes_per_launch": alg_bytes, "avg_ms": dd_ms,
                      "note": ("algorithmic bytes = fp16 subband images read once "
                               f"({ring_bytes}) + float64 mean/m2 read+write ({stat_bytes}); the "
-                              "kernel is VALU/LDS-bound: ~26 fp32 ops and 16 LDS reads per (DM, step, pixel)")},
+                              "kernel is LDS-bandwidth-bound, not HBM-bound: per (DM, step, pixel) 16 float32 "
+                              "window reads (8.7 LDS and 27.9 VALU instructions per 64 lanes, hardware "
+                              "counters), LDS ~95% busy in the compute phase")},
     }
     ops = cfg.n_dm * cfg.n_steps * n_pix * (cfg.n_sub + 3 * len(cfg.widths) + 4)
     out["roofline"]["valu_tops"] = ops / (dd_ms * 1e-3) / 1e12

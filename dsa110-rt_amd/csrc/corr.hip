@@ -28,6 +28,7 @@ struct CorrParams {
     const int8_t *im8;
     int32_t *vis;
     int n_pkt, n_ant, n_chan, t_int, csf, fuse_pol, in_layout, out_layout;
+    int vec4;   // raw input read 4 channels (16 bytes) at a time: csf, n_chan multiples of 4
     int n_steps, n_cout, nbase, kq, row_stride, n_tiles, n_pairs, n_items;
 };
 
@@ -58,11 +59,60 @@ __global__ __launch_bounds__(CORR_THREADS) void corr_i8_kernel(CorrParams P) {
     const int wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, li = lane & 15;
 
+    // vec4 unpack: thread -> (packet, antenna, 4-channel group) slots, stepped by
+    // CORR_THREADS slots with carries instead of divisions
+    const int q4 = P.csf >> 2;
+    const int d_grp = P.vec4 ? CORR_THREADS % q4 : 0, d_row = P.vec4 ? CORR_THREADS / q4 : 0;
+    const int grp0 = P.vec4 ? tid % q4 : 0, row0 = P.vec4 ? tid / q4 : 0;
+    const int ant0 = P.vec4 ? row0 % P.n_ant : 0, pkl0 = P.vec4 ? row0 / P.n_ant : 0;
+
     for (int item = blockIdx.x; item < P.n_items; item += gridDim.x) {
         const int step = item / P.n_cout;
         const int co = item - step * P.n_cout;
         // ---- unpack this (step, channel group) into LDS rows
-        if (P.in_layout == 0) {
+        if (P.vec4) {
+            // 4 channels per 16-byte load; per part 8 bytes (4 channels x 2 steps) per LDS store
+            int grp = grp0, ant = ant0, pkt_l = pkl0;
+            while (pkt_l < ppf) {
+                const int pkt = step * ppf + pkt_l;
+                const uint4 w4 = *reinterpret_cast<const uint4 *>(
+                    P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + co * P.csf + 4 * grp) << 2));
+                const uint32_t wv[4] = {w4.x, w4.y, w4.z, w4.w};
+                uint32_t r0[2], r1[2], m0[2], m1[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    uint32_t pr0[2], pr1[2], pm0[2], pm1[2];
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const uint32_t word = wv[2 * h + c];      // bytes: (t0 p0, t0 p1, t1 p0, t1 p1)
+                        const uint32_t r = sext_nib4(word & 0x0F0F0F0Fu);
+                        const uint32_t m = sext_nib4((word >> 4) & 0x0F0F0F0Fu);
+                        pr0[c] = (r & 0xFFu) | ((r >> 8) & 0xFF00u);
+                        pr1[c] = ((r >> 8) & 0xFFu) | ((r >> 16) & 0xFF00u);
+                        pm0[c] = (m & 0xFFu) | ((m >> 8) & 0xFF00u);
+                        pm1[c] = ((m >> 8) & 0xFFu) | ((m >> 16) & 0xFF00u);
+                    }
+                    r0[h] = pr0[0] | (pr0[1] << 16);
+                    r1[h] = pr1[0] | (pr1[1] << 16);
+                    m0[h] = pm0[0] | (pm0[1] << 16);
+                    m1[h] = pm1[0] | (pm1[1] << 16);
+                }
+                const int k = (pkt_l * P.csf + 4 * grp) * 2;           // 8-byte aligned
+                int8_t *row = lds + ant * P.row_stride;
+                *reinterpret_cast<uint2 *>(row + k) = make_uint2(r0[0], r0[1]);
+                *reinterpret_cast<uint2 *>(row + kq + k) = make_uint2(r1[0], r1[1]);
+                *reinterpret_cast<uint2 *>(row + 2 * kq + k) = make_uint2(m0[0], m0[1]);
+                *reinterpret_cast<uint2 *>(row + 3 * kq + k) = make_uint2(m1[0], m1[1]);
+                grp += d_grp;
+                const int carry = grp >= q4 ? 1 : 0;
+                grp -= carry * q4;
+                ant += d_row + carry;
+                while (ant >= P.n_ant) {
+                    ant -= P.n_ant;
+                    ++pkt_l;
+                }
+            }
+        } else if (P.in_layout == 0) {
             const int n_words = ppf * P.n_ant * P.csf;
             for (int w = tid; w < n_words; w += CORR_THREADS) {
                 const int ch_l = w % P.csf;
@@ -236,6 +286,7 @@ extern "C" int dsx_corr_i8(const void *raw, const void *im, int32_t *vis, int n_
     P.fuse_pol = fuse_pol ? 1 : 0;
     P.in_layout = input_layout;
     P.out_layout = out_layout;
+    P.vec4 = input_layout == 0 && csf % 4 == 0 && n_chan % 4 == 0 && ((uintptr_t)raw & 15) == 0;
     P.n_steps = 2 * n_pkt / t_int;
     P.n_cout = n_chan / csf;
     P.nbase = n_ant * (n_ant + 1) / 2;
