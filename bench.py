@@ -29,6 +29,7 @@ import numpy as np
 import torch
 
 HBM_PEAK_GBPS = 8000.0
+LDS_PEAK_TBPS = 256 * 128 * 2.4e9 / 1e12     # MI355X LDS read bandwidth (128 B/clk/CU)
 MFMA_I8_PEAK_TOPS = 5000.0
 FP32_PEAK_TFLOPS = 157.3
 
@@ -320,6 +321,12 @@ def main():
     }
     ops = cfg.n_dm * cfg.n_steps * n_pix * (cfg.n_sub + 3 * len(cfg.widths) + 4)
     out["roofline"]["valu_tops"] = ops / (dd_ms * 1e-3) / 1e12
+    # the binding resource: LDS reads of the float32 subband windows, 4 B per subband term of
+    # every dedispersed sample, against 128 B/clk/CU x 256 CUs x 2.4 GHz
+    lds_bytes = 4 * cfg.n_sub * cfg.n_dm * cfg.n_steps * n_pix
+    lds_tbps = lds_bytes / (dd_ms * 1e-3) / 1e12
+    out["roofline"]["lds"] = {"bytes_per_launch": lds_bytes, "achieved_TBps": lds_tbps,
+                              "peak_TBps": LDS_PEAK_TBPS, "frac": lds_tbps / LDS_PEAK_TBPS}
     del node
     if not args.no_plain:
         torch.cuda.empty_cache()

@@ -18,8 +18,10 @@
 namespace dsx {
 
 constexpr int IMG_THREADS = 256;
-constexpr int IMG_MAX_KB = 32;          // <= 64 rows per chunk (2 rows = 4 K values per MFMA)
-constexpr int IMG_TAP_BATCH = 24;        // taps per thread with their loads in flight together
+constexpr int IMG_MAX_KB = 20;          // <= 40 rows per chunk (2 rows = 4 K values per MFMA)
+constexpr int IMG_TAP_BATCH = 24;
+constexpr int IMG_KB_GROUP = 4;         // column-transform B operands loaded per group of MFMAs
+static_assert(IMG_MAX_KB % IMG_KB_GROUP == 0, "K-block groups must tile IMG_MAX_KB");        // taps per thread with their loads in flight together
 constexpr int TAP_CONJ = 1 << 20;
 constexpr int TAP_NEG = 1 << 21;
 constexpr int TAP_CELL = (1 << 20) - 1;
@@ -271,24 +273,45 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                         a_op[kb] = a;
                     }
                 }
-                for (int jt = 0; jt < n_tiles_1d; ++jt) {
-                    const int j0 = jt * 16;
-                    v4f acc = {0.f, 0.f, 0.f, 0.f};
+                // two output tiles (jt, jt + 1) per pass: independent accumulator chains
+                // interleave, so one tile's MFMA latency hides behind the other's
+                for (int jt = 0; jt < n_tiles_1d; jt += 2) {
+                    const bool two = jt + 1 < n_tiles_1d;
+                    v4f acc2[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+                    // B operands in groups of IMG_KB_GROUP: every LDS load of a group is
+                    // issued (clamped row and tile, in bounds) before the group's MFMAs
 #pragma unroll
-                    for (int kb = 0; kb < IMG_MAX_KB; ++kb) {
-                        if (kb < nkb) {
-                            const int k = kb * 4 + lg;
-                            const int r = k >> 1;
-                            float bv = 0.f;
-                            if (r < rc) {
-                                const int rg = r0 + r;
-                                const float cr = (rg == 0 || rg == halfN) ? 1.f : 2.f;
-                                const float2 y = H[r * RS + j0 + lr];
-                                bv = cr * ((k & 1) ? y.y : y.x);
+                    for (int kb0 = 0; kb0 < IMG_MAX_KB; kb0 += IMG_KB_GROUP) {
+                        if (kb0 < nkb) {
+                            float bv[2][IMG_KB_GROUP];
+#pragma unroll
+                            for (int h = 0; h < 2; ++h) {
+                                const int jc = (h == 0 || two) ? (jt + h) * 16 : jt * 16;
+#pragma unroll
+                                for (int u = 0; u < IMG_KB_GROUP; ++u) {
+                                    const int k = (kb0 + u) * 4 + lg;
+                                    const int r = k >> 1;
+                                    const bool ok = kb0 + u < nkb && r < rc;
+                                    const int rr = ok ? r : 0;
+                                    const int rg = r0 + rr;
+                                    const float cr = ok ? ((rg == 0 || rg == halfN) ? 1.f : 2.f) : 0.f;
+                                    const float2 y = H[rr * RS + jc + lr];
+                                    bv[h][u] = cr * ((k & 1) ? y.y : y.x);
+                                }
                             }
-                            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[kb], bv, acc, 0, 0, 0);
+#pragma unroll
+                            for (int u = 0; u < IMG_KB_GROUP; ++u)
+                                if (kb0 + u < nkb) {
+                                    acc2[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[kb0 + u], bv[0][u], acc2[0], 0, 0, 0);
+                                    acc2[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[kb0 + u], bv[1][u], acc2[1], 0, 0, 0);
+                                }
                         }
                     }
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                    if (h == 1 && !two) break;
+                    const int j0 = (jt + h) * 16;
+                    const v4f acc = acc2[h];
                     // ---- 4. epilogue
                     const int j = j0 + lr;
 #pragma unroll
@@ -300,6 +323,7 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                         const int64_t o = (int64_t)i * N + j;
                         if (ck > 0) val += ld_as_float(out, o);
                         st_from_float(out, o, val);
+                    }
                     }
                 }
             }
