@@ -315,14 +315,15 @@ def main():
                      "alg_bytes_per_launch": alg_bytes, "avg_ms": dd_ms,
                      "note": ("algorithmic bytes = fp16 subband images read once "
                               f"({ring_bytes}) + float64 mean/m2 read+write ({stat_bytes}); the "
-                              "kernel is LDS-bandwidth-bound, not HBM-bound: per (DM, step, pixel) 16 float32 "
-                              "window reads (8.7 LDS and 27.9 VALU instructions per 64 lanes, hardware "
-                              "counters), LDS ~95% busy in the compute phase")},
+                              "kernel is not HBM-bound: per (DM, step, pixel) 16 float32 window reads from LDS "
+                              "and ~28 VALU instructions (hardware counters: 27.9 VALU and 8.7 LDS instructions "
+                              "per 64 lanes); VALU issue ~80% in the compute phase, window staging (~10 ms) "
+                              "not overlapped")},
     }
     ops = cfg.n_dm * cfg.n_steps * n_pix * (cfg.n_sub + 3 * len(cfg.widths) + 4)
     out["roofline"]["valu_tops"] = ops / (dd_ms * 1e-3) / 1e12
-    # the binding resource: LDS reads of the float32 subband windows, 4 B per subband term of
-    # every dedispersed sample, against 128 B/clk/CU x 256 CUs x 2.4 GHz
+    # LDS reads of the float32 subband windows, 4 B per subband term of every dedispersed
+    # sample, against 128 B/clk/CU x 256 CUs x 2.4 GHz
     lds_bytes = 4 * cfg.n_sub * cfg.n_dm * cfg.n_steps * n_pix
     lds_tbps = lds_bytes / (dd_ms * 1e-3) / 1e12
     out["roofline"]["lds"] = {"bytes_per_launch": lds_bytes, "achieved_TBps": lds_tbps,
