@@ -20,6 +20,7 @@ namespace dsx {
 constexpr int IMG_THREADS = 256;
 constexpr int IMG_MAX_KB = 20;          // <= 40 rows per chunk (2 rows = 4 K values per MFMA)
 constexpr int IMG_TAP_BATCH = 24;
+constexpr int IMG_JT = 2;               // output tiles per column-transform pass
 constexpr int IMG_KB_GROUP = 4;         // column-transform B operands loaded per group of MFMAs
 static_assert(IMG_MAX_KB % IMG_KB_GROUP == 0, "K-block groups must tile IMG_MAX_KB");        // taps per thread with their loads in flight together
 constexpr int TAP_CONJ = 1 << 20;
@@ -273,20 +274,21 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                         a_op[kb] = a;
                     }
                 }
-                // two output tiles (jt, jt + 1) per pass: independent accumulator chains
-                // interleave, so one tile's MFMA latency hides behind the other's
-                for (int jt = 0; jt < n_tiles_1d; jt += 2) {
-                    const bool two = jt + 1 < n_tiles_1d;
-                    v4f acc2[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+                // IMG_JT output tiles per pass: independent accumulator chains interleave,
+                // so one tile's MFMA latency hides behind the others'
+                for (int jt = 0; jt < n_tiles_1d; jt += IMG_JT) {
+                    v4f acc2[IMG_JT];
+#pragma unroll
+                    for (int h = 0; h < IMG_JT; ++h) acc2[h] = v4f{0.f, 0.f, 0.f, 0.f};
                     // B operands in groups of IMG_KB_GROUP: every LDS load of a group is
                     // issued (clamped row and tile, in bounds) before the group's MFMAs
 #pragma unroll
                     for (int kb0 = 0; kb0 < IMG_MAX_KB; kb0 += IMG_KB_GROUP) {
                         if (kb0 < nkb) {
-                            float bv[2][IMG_KB_GROUP];
+                            float bv[IMG_JT][IMG_KB_GROUP];
 #pragma unroll
-                            for (int h = 0; h < 2; ++h) {
-                                const int jc = (h == 0 || two) ? (jt + h) * 16 : jt * 16;
+                            for (int h = 0; h < IMG_JT; ++h) {
+                                const int jc = (jt + h < n_tiles_1d ? jt + h : jt) * 16;
 #pragma unroll
                                 for (int u = 0; u < IMG_KB_GROUP; ++u) {
                                     const int k = (kb0 + u) * 4 + lg;
@@ -302,14 +304,15 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
 #pragma unroll
                             for (int u = 0; u < IMG_KB_GROUP; ++u)
                                 if (kb0 + u < nkb) {
-                                    acc2[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[kb0 + u], bv[0][u], acc2[0], 0, 0, 0);
-                                    acc2[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[kb0 + u], bv[1][u], acc2[1], 0, 0, 0);
+#pragma unroll
+                                    for (int h = 0; h < IMG_JT; ++h)
+                                        acc2[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[kb0 + u], bv[h][u], acc2[h], 0, 0, 0);
                                 }
                         }
                     }
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                    if (h == 1 && !two) break;
+                    for (int h = 0; h < IMG_JT; ++h) {
+                    if (jt + h >= n_tiles_1d) break;
                     const int j0 = (jt + h) * 16;
                     const v4f acc = acc2[h];
                     // ---- 4. epilogue
