@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--history", choices=("fp16", "fp32"), default="fp32",
+                    help="image history dtype of the fused form (fp32: LDS-direct window loads)")
     ap.add_argument("--form", choices=["chosen", "plain"], default="chosen")
     ap.add_argument("--dm-plan", choices=["even", "uneven"], default="even",
                     help="uneven: 96-antenna 384-channel inputs with the three-range DM plan (1x/2x/4x, widths 1-64)")
@@ -179,7 +181,7 @@ def serve_main(args):
                                   f"96 ant x 384 ch, Poisson arrivals, 200 GB HBM budget",
                       "data_seconds": float(sum(durs))}, "rates": {}}
     for rate in [float(r) for r in args.serve_rates.split(",")]:
-        node = SearchNode(cfg, sc.antpos_e, sc.antpos_n, ring_dtype=torch.float16, max_block_steps=32768 // 8,
+        node = SearchNode(cfg, sc.antpos_e, sc.antpos_n, max_block_steps=32768 // 8,
                           n_slots=2)
         server = NodeServer([node], hbm_budget_bytes=200 << 30)
         arr = poisson_arrivals(durs, rate, 0xA77 + int(rate * 10))
@@ -274,7 +276,8 @@ def main():
     blocks = [gs.block(i * cfg.n_samp, cfg.n_samp) for i in range(n_blocks)]
     torch.cuda.synchronize()
 
-    chosen = run_form(cfg, sc, blocks, args.steps, args.warmup, torch.float16)
+    hist_dt = torch.float16 if args.history == "fp16" else torch.float32
+    chosen = run_form(cfg, sc, blocks, args.steps, args.warmup, hist_dt)
     node = chosen.pop("node")
     wall = chosen["wall"]
     rtf = args.steps * cfg.block_seconds / wall
@@ -282,13 +285,16 @@ def main():
     # dominant kernel: fused dedispersion + detection
     dd_ms = chosen["stage_ms"]["dedisp"]
     n_pix = cfg.n_grid * cfg.n_grid
-    ring_bytes = cfg.n_sub * cfg.n_steps * n_pix * 2
+    ring_bytes = cfg.n_sub * cfg.n_steps * n_pix * (2 if args.history == "fp16" else 4)
     stat_bytes = 2 * 2 * n_pix * cfg.n_dm * 8
     alg_bytes = ring_bytes + stat_bytes
     achieved = alg_bytes / (dd_ms * 1e-3) / 1e9
     plan = node.plans[cfg.n_steps][0]
     maxt = 8 if plan.n_terms <= 8 else 16 if plan.n_terms <= 16 else 0
-    kname = f"dedisp_kernel<_Float16, {len(cfg.widths)}, {str(plan.paired).lower()}, {maxt}>"
+    is_async = plan.q_total > 0 and not plan.paired
+    tname = "_Float16" if args.history == "fp16" else "float"
+    kname = (f"dedisp_kernel<{tname}, {len(cfg.widths)}, {str(plan.paired).lower()}, {maxt}"
+             + (", true>" if is_async else ">"))
     out = {
         "metric": "real-time factor + p99 block->candidate latency, 96-ant 384-ch node; chosen vs plain",
         "value": rtf,
@@ -300,14 +306,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8->int32 (corr), fp32 FFT, fp16 store / fp32 acc, fp64 stats",
+        "dtype": f"int8->int32 (corr), fp32 FFT, {args.history} store / fp32 acc, fp64 stats",
         "data": "synthetic (seeded 4-bit noise + 8 dispersed bursts, generated on the GPU)",
         "config": {"workload": workload(cfg), "form": "chosen", "block_ms": cfg.block_seconds * 1e3},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3),
                        "block_duration": cfg.block_seconds * 1e3},
         "stages_ms": chosen["stage_ms"],
         "candidates": chosen["n_cands"],
-        "dedisp_layout": {"tile": plan.tile, "paired": plan.paired, "lds_bytes": plan.lds_bytes,
+        "dedisp_layout": {"tile": plan.tile, "paired": plan.paired, "async": is_async, "lds_bytes": plan.lds_bytes,
                           "autotune_ms": chosen["tune"]},
         "roofline": {"kernel": f"dsx_dedisp_detect ({kname})", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",

@@ -117,16 +117,16 @@ __device__ __forceinline__ void sum_terms(float (&Pw)[NP], const E *X, const uin
     }
 }
 
-template <typename T, int NW, bool PAIRED, int MAXT>
+template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false>
 __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
     constexpr int WMAX = 1 << (NW - 1);
     constexpr int H2 = WMAX / 2;
     constexpr int WW = 2 * H2;
     constexpr int NP = DD_SUB + WW + 1;              // register prefix window
     extern __shared__ float lds[];
-    float *Q = lds;                                   // [q_total] float32 pair rows
-    float *R = lds + P.q_total;                       // [r_total] subband windows
-    const float *X = PAIRED ? Q : R;                  // rows the lanes sum
+    float *Q = lds;                                   // [q_total] float32 pair rows (ASYNC: window buffer 1)
+    float *R = lds + P.q_total;                       // [r_total] subband windows (ASYNC: window buffer 0)
+    const float *X = PAIRED ? Q : R;                  // rows the lanes sum (ASYNC: this tile's buffer)
     const T *ring = reinterpret_cast<const T *>(P.ring);
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
@@ -161,6 +161,38 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
     bool fast = MAXT > 0 && P.n_sub <= n_waves * DD_STAGE_SUBS;
     for (int s = 0; s < P.n_sub; ++s) fast = fast && si[3 * s + 2] <= 64 * DD_STAGE_ITERS;
 
+    // ASYNC (float32 history, unpaired plan with two window buffers): the
+    // windows of the next tile -- same pixel, or the first tile of the
+    // work-group's next pixel -- are copied from HBM into the other buffer by
+    // LDS-direct loads (global_load_lds_dword: lane l fills slot base + l, no
+    // registers hold the data) while the current tile computes.  Steps outside
+    // the history are zero-filled by ordinary stores.
+    auto dma_stage = [&](int pp, int cc0, float *dst) {
+        const int64_t pos0 = (int64_t)P.hist + (P.c_lo + cc0) - H2 - P.T0;
+        const float *ring_p = reinterpret_cast<const float *>(P.ring) + (int64_t)pp * P.row_len;
+        for (int s = wave; s < P.n_sub; s += n_waves) {
+            const int off = si[3 * s];
+            const int len = si[3 * s + 2];
+            const int64_t q0 = pos0 - si[3 * s + 1];
+            const float *row = ring_p + (int64_t)s * plane;
+            for (int e0 = 0; e0 < len; e0 += 64) {
+                const int e = e0 + lane;
+                const int64_t pos = q0 + e;
+                const bool in = e < len;
+                if (in && pos >= 0 && pos < P.row_len)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void *)(row + pos),
+                        (__attribute__((address_space(3))) void *)(dst + off + e0), 4, 0, 0);
+                else if (in)
+                    dst[off + e] = 0.f;
+            }
+        }
+    };
+    int tile_no = 0;
+    if constexpr (ASYNC) {
+        if ((int)blockIdx.x < P.n_pix) dma_stage(blockIdx.x, 0, R);
+    }
+
     for (int p = blockIdx.x; p < P.n_pix; p += gridDim.x) {
         // ---- per-lane state from earlier blocks (float64)
         // w_k mean and 1 / (sigma sqrt(w_k)) are power-of-two scalings of the
@@ -190,6 +222,20 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
             const int64_t cabs0 = P.c_lo + c0;
             // ---- stage every subband's window (ring dtype): step tau of subband s
             //      lands at R[off_s + tau - (cabs0 - H2 - shmax_s)]
+            if constexpr (ASYNC) {
+                // this tile's loads were issued one tile ago: wait for this wave's,
+                // then for every wave's (and for every wave to leave the other buffer)
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+                X = (tile_no & 1) ? Q : R;
+                int np = p, nc = c0 + P.tt;
+                if (nc >= P.n_centres) {
+                    np = p + (int)gridDim.x;
+                    nc = 0;
+                }
+                if (np < P.n_pix) dma_stage(np, nc, (tile_no & 1) ? R : Q);
+                ++tile_no;
+            } else {
             __syncthreads();
             const int64_t pos0 = (int64_t)P.hist + cabs0 - H2 - P.T0;   // ring position of e - shmax_s = 0
             const T *ring_p = ring + (int64_t)p * P.row_len;
@@ -250,6 +296,7 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                 }
             }
             __syncthreads();
+            }
             // ---- one float32 row per (pair, delta): Q[e] = R[a0 + e] + R[b0 + e];
             //      each wave builds every n_waves-th row
             if (PAIRED) {
@@ -432,12 +479,33 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
 }
 
 // Plans with at most 8 (or 16) terms keep the lanes' term bases in registers.
-template <typename T, int NW, bool PAIRED, int MAXT>
+template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false>
 static int launch_maxt(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
-    (void)hipFuncSetAttribute((const void *)dedisp_kernel<T, NW, PAIRED, MAXT>,
+    (void)hipFuncSetAttribute((const void *)dedisp_kernel<T, NW, PAIRED, MAXT, ASYNC>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    dedisp_kernel<T, NW, PAIRED, MAXT><<<grid, threads, lds, s>>>(P);
+    dedisp_kernel<T, NW, PAIRED, MAXT, ASYNC><<<grid, threads, lds, s>>>(P);
     return check_launch("dsx_dedisp_detect");
+}
+
+// float32 history, unpaired plan with two window buffers
+template <int NW>
+static int launch_async(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
+    if (P.n_terms <= 8) return launch_maxt<float, NW, false, 8, true>(P, threads, lds, grid, s);
+    if (P.n_terms <= 16) return launch_maxt<float, NW, false, 16, true>(P, threads, lds, grid, s);
+    return launch_maxt<float, NW, false, 0, true>(P, threads, lds, grid, s);
+}
+
+static int dispatch_async(int nw, const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
+    switch (nw) {
+        case 1: return launch_async<1>(P, threads, lds, grid, s);
+        case 2: return launch_async<2>(P, threads, lds, grid, s);
+        case 3: return launch_async<3>(P, threads, lds, grid, s);
+        case 4: return launch_async<4>(P, threads, lds, grid, s);
+        case 5: return launch_async<5>(P, threads, lds, grid, s);
+        case 6: return launch_async<6>(P, threads, lds, grid, s);
+        case 7: return launch_async<7>(P, threads, lds, grid, s);
+        default: return set_error(DSX_ENOTSUP, "dsx_dedisp_detect: n_widths=%d not in [1, 7]", nw);
+    }
 }
 
 template <typename T, int NW, bool PAIRED>
@@ -516,7 +584,9 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
     DSX_REQUIRE(n_centres > 0 && n_centres % DD_CHUNK == 0,
                 "dsx_dedisp_plan: n_centres must be a positive multiple of 64");
     DSX_REQUIRE(ring_dtype == DSX_F16 || ring_dtype == DSX_F32, "dsx_dedisp_plan: ring_dtype");
-    DSX_REQUIRE(mode >= 0 && mode <= 2, "dsx_dedisp_plan: mode=%d not in {0, 1, 2}", mode);
+    DSX_REQUIRE(mode >= 0 && mode <= 3, "dsx_dedisp_plan: mode=%d not in {0, 1, 2, 3}", mode);
+    DSX_REQUIRE(mode != 3 || ring_dtype == DSX_F32,
+                "dsx_dedisp_plan: mode 3 (LDS-direct window loads) needs a float32 history");
     const int ww = n_widths > 1 ? (1 << (n_widths - 1)) : 0;
     const int h2 = ww / 2;
     std::vector<int> shmax(n_sub), shmin(n_sub);
@@ -544,11 +614,11 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
     // largest tile within the LDS budget, preferring tiles whose windows fit
     // the batched staging of the fast kernel
     const int threads = std::min(DD_THREADS, ((n_dm + 63) / 64) * 64);
-    auto best_tile = [&](bool paired) {
+    auto best_tile = [&](bool paired, bool dbl) {
         int best = 0, best_fast = 0;
         const int n_terms = paired ? (n_sub + 1) / 2 : n_sub;
         for (int t = DD_CHUNK; t <= n_centres && t <= 2048; t += DD_CHUNK) {
-            const int64_t bytes = (paired ? q_elems(t) * 4 : 0) + r_elems(t) * 4 +
+            const int64_t bytes = (paired ? q_elems(t) * 4 : 0) + r_elems(t) * 4 * (dbl ? 2 : 1) +
                                   ((int64_t)tb_words(n_terms) * threads + 3 * n_sub) * 4;
             if (n_centres % t != 0 || bytes > lds_budget) continue;
             best = t;
@@ -560,15 +630,15 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
     };
     bool paired = false;
     int tt = 0;
-    if (mode != 2 && n_prow <= DD_MAX_ROWS) {
-        tt = best_tile(true);
+    if (mode != 2 && mode != 3 && n_prow <= DD_MAX_ROWS) {
+        tt = best_tile(true, false);
         paired = tt > 0;
     }
     if (!paired) {
         if (mode == 1)
             return set_error(DSX_ERANGE, "dsx_dedisp_plan: pair rows need %lld B of LDS > %d",
                              (long long)(q_elems(DD_CHUNK) * 4 + r_elems(DD_CHUNK) * 4), lds_budget);
-        tt = best_tile(false);
+        tt = best_tile(false, mode == 3);
     }
     if (tt == 0)
         return set_error(DSX_ERANGE, "dsx_dedisp_plan: the 64-centre tile needs %lld B of LDS > %d",
@@ -587,7 +657,7 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
         sub_info[3 * s + 2] = len;
         r_total += len;
     }
-    int q_total = 0;
+    int q_total = mode == 3 ? r_total : 0;     // mode 3: the second window buffer
     if (paired) {
         // row r covers tau in [tile start - W/2 - hi_r, + len_r) of its pair's DMs
         std::vector<int> qoff(n_prow);
@@ -652,8 +722,9 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
                 "dsx_dedisp_detect: bad shape");
     DSX_REQUIRE(n_dm <= 32767, "dsx_dedisp_detect: n_dm=%d > 32767", n_dm);
     const bool paired = n_rows > 0;
+    const bool async = !paired && q_total > 0;
     DSX_REQUIRE(paired ? (rows && n_terms == (n_sub + 1) / 2 && q_total > 0 && n_rows <= DD_MAX_ROWS)
-                       : (n_terms == n_sub && q_total == 0),
+                       : (n_terms == n_sub && (q_total == 0 || (q_total == r_total && ring_dtype == DSX_F32))),
                 "dsx_dedisp_detect: n_rows=%d / n_terms=%d / q_total=%d do not describe a plan for n_sub=%d",
                 n_rows, n_terms, q_total, n_sub);
     DSX_REQUIRE(n_centres > 0 && n_centres % DD_CHUNK == 0,
@@ -697,6 +768,7 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
     P.min_count = min_count;
     const dim3 grid((unsigned)std::min(n_pix, 1 << 20), (unsigned)((n_dm + DD_THREADS - 1) / DD_THREADS));
     hipStream_t s = as_stream(stream);
+    if (async) return dispatch_async(n_widths, P, threads, lds, grid, s);
     if (ring_dtype == DSX_F16)
         return paired ? dispatch_nw<_Float16, true>(n_widths, P, threads, lds, grid, s)
                       : dispatch_nw<_Float16, false>(n_widths, P, threads, lds, grid, s);

@@ -85,6 +85,55 @@ __global__ __launch_bounds__(256) void ring_transpose16_kernel(const _Float16 *i
     }
 }
 
+// float32 history with 16-byte accesses (n_pix, n_new multiples of 64; hist, row_len
+// multiples of 4): tile rows read as 4-pixel vectors, each thread writes 4
+// consecutive steps of one pixel.
+__global__ __launch_bounds__(256) void ring_transpose16f_kernel(const float *img, float *ring, int n_sub,
+                                                                int n_pix, int n_new, int hist, int row_len) {
+    __shared__ __align__(16) float tile[64][68];        // [step][pixel], 272-byte rows
+    const int tid = threadIdx.x;
+    const int tiles_p = n_pix / 64, tiles_t = n_new / 64;
+    const int64_t n_tiles = (int64_t)n_sub * tiles_p * tiles_t;
+    for (int64_t b = blockIdx.x; b < n_tiles; b += gridDim.x) {
+        const int s = (int)(b / ((int64_t)tiles_p * tiles_t));
+        const int rem = (int)(b - (int64_t)s * tiles_p * tiles_t);
+        const int tt = rem / tiles_p, tp = rem - tt * tiles_p;
+        const int t0 = tt * 64, p0 = tp * 64;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = tid + 256 * k;
+            const int r = i >> 4, j = i & 15;
+            *reinterpret_cast<float4 *>(&tile[r][4 * j]) =
+                *reinterpret_cast<const float4 *>(img + ((int64_t)s * n_new + t0 + r) * n_pix + p0 + 4 * j);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = tid + 256 * k;
+            const int q = i & 63, c = i >> 6;
+            *reinterpret_cast<float4 *>(ring + ((int64_t)s * n_pix + p0 + q) * row_len + hist + t0 + 4 * c) =
+                make_float4(tile[4 * c][q], tile[4 * c + 1][q], tile[4 * c + 2][q], tile[4 * c + 3][q]);
+        }
+    }
+}
+
+// History carry in 16-byte pieces (hist, row_len, prev_len multiples of 16 bytes' worth of elements).
+template <typename T>
+__global__ __launch_bounds__(256) void ring_hist_vec_kernel(const T *old_ring, T *new_ring, int64_t n_rows,
+                                                            int hist, int row_len, int prev_len) {
+    constexpr int V = 16 / sizeof(T);
+    const int hv = hist / V;
+    const int64_t n = n_rows * hv;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / hv;
+        const int k = (int)(i - row * hv);
+        *reinterpret_cast<uint4 *>(new_ring + row * row_len + V * k) =
+            *reinterpret_cast<const uint4 *>(old_ring + row * row_len + prev_len + V * k);
+    }
+}
+
 // History carry in 16-byte pieces (hist, row_len, prev_len multiples of 8).
 __global__ __launch_bounds__(256) void ring_hist16_kernel(const _Float16 *old_ring, _Float16 *new_ring,
                                                           int64_t n_rows, int hist, int row_len, int prev_len) {
@@ -142,7 +191,17 @@ extern "C" int dsx_ring_advance(const void *ring_old, void *ring_new, const void
     const bool vec16 = dtype == DSX_F16 && n_pix % 64 == 0 && n_new % 64 == 0 && hist % 8 == 0 &&
                        row_len % 8 == 0 && prev_len % 8 == 0 && ((uintptr_t)img & 15) == 0 &&
                        ((uintptr_t)ring_old & 15) == 0 && ((uintptr_t)ring_new & 15) == 0;
-    if (vec16) {
+    const bool vec16f = dtype == DSX_F32 && n_pix % 64 == 0 && n_new % 64 == 0 && hist % 4 == 0 &&
+                        row_len % 4 == 0 && prev_len % 4 == 0 && ((uintptr_t)img & 15) == 0 &&
+                        ((uintptr_t)ring_old & 15) == 0 && ((uintptr_t)ring_new & 15) == 0;
+    if (vec16f) {
+        const int64_t nv = n_rows * (hist / 4);
+        if (nv > 0)
+            ring_hist_vec_kernel<float><<<(int)std::min<int64_t>((nv + 255) / 256, 65536), 256, 0, s>>>(
+                (const float *)ring_old, (float *)ring_new, n_rows, hist, row_len, prev_len);
+        ring_transpose16f_kernel<<<tb, 256, 0, s>>>((const float *)img, (float *)ring_new, n_sub, n_pix, n_new,
+                                                    hist, row_len);
+    } else if (vec16) {
         const int64_t nv = n_rows * (hist / 8);
         if (nv > 0)
             ring_hist16_kernel<<<(int)std::min<int64_t>((nv + 255) / 256, 65536), 256, 0, s>>>(

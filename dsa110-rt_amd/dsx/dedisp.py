@@ -203,7 +203,10 @@ class DedispPlan:
 def plan_dedisp(shift_sd: np.ndarray, *, n_widths: int, n_centres: int, ring_dtype: torch.dtype,
                 lds_budget: int, mode: int = 0) -> DedispPlan:
     """Run ``dsx_dedisp_plan`` (host code, no GPU) for ``shift_sd`` = [n_sub, n_dm].
-    ``mode`` 0 = pair rows when they fit, 1 = pair rows only, 2 = unpaired."""
+    ``mode`` 0 = pair rows when they fit, 1 = pair rows only, 2 = unpaired,
+    3 = unpaired with two window buffers, the next tile's windows loaded from
+    HBM straight into LDS while the current tile computes (float32 history;
+    ``q_total`` = ``r_total`` marks the second buffer)."""
     sh = np.ascontiguousarray(shift_sd, dtype=np.int32)
     if sh.ndim != 2:
         raise ValueError("shift_sd must be [n_sub, n_dm]")

@@ -31,6 +31,7 @@ from .pattern import build_cell_map, build_imager_taps
 
 #: LDS per dedispersion work-group: two 256-lane work-groups per CU.
 LDS_BUDGET_DEDISP = 80 * 1024
+LDS_BUDGET_ASYNC = 48 * 1024
 
 
 @dataclass
@@ -47,10 +48,11 @@ class SearchNode:
     """Stateful node pipeline; state (history ring, statistics) stays on the GPU."""
 
     def __init__(self, cfg: NodeConfig, antpos_e: np.ndarray, antpos_n: np.ndarray, *,
-                 device: str = "cuda", ring_dtype: torch.dtype = torch.float16,
+                 device: str = "cuda", ring_dtype: torch.dtype = torch.float32,
                  max_block_steps: Optional[int] = None, cand_cap: int = 1 << 20,
-                 stream: Optional[torch.cuda.Stream] = None, lds_budget: int = LDS_BUDGET_DEDISP,
-                 dedisp_mode: int = 2, n_slots: int = 1, with_front: bool = True, with_detect: bool = True):
+                 stream: Optional[torch.cuda.Stream] = None, lds_budget: Optional[int] = None,
+                 dedisp_mode: Optional[int] = None, n_slots: int = 1, with_front: bool = True,
+                 with_detect: bool = True):
         cfg.validate()
         if ring_dtype not in (torch.float16, torch.float32):
             raise TypeError("ring_dtype must be float16 or float32")
@@ -121,6 +123,14 @@ class SearchNode:
         self._cnt_pinned = [torch.zeros(1, dtype=torch.int32, pin_memory=True) for _ in range(self.n_slots)]
         self._rec_pinned = [torch.zeros((self._rec_prefix, 4), dtype=torch.int32, pin_memory=True)
                             for _ in range(self.n_slots)]
+        # default layout: float32 history -> two window buffers filled by LDS-direct
+        # loads (tile 128 within 48 KB); fp16 history -> one unpaired buffer (80 KB)
+        if dedisp_mode is None:
+            dedisp_mode = 3 if ring_dtype == torch.float32 else 2
+        if dedisp_mode == 3 and ring_dtype != torch.float32:
+            raise ValueError("dedisp_mode 3 (LDS-direct window loads) needs a float32 history")
+        if lds_budget is None:
+            lds_budget = LDS_BUDGET_ASYNC if dedisp_mode == 3 else LDS_BUDGET_DEDISP
         self.lds_budget = int(lds_budget)
         self.dedisp_mode = int(dedisp_mode)
         self.plans: Dict[int, tuple] = {}
@@ -148,13 +158,21 @@ class SearchNode:
     def _plan_for(self, n_centres: int):
         """Device copy of the dedispersion LDS plan for blocks of ``n_centres`` steps."""
         if n_centres not in self.plans:
-            kw = dict(n_widths=self.n_widths, n_centres=n_centres, ring_dtype=self.ring_dtype,
-                      mode=self.dedisp_mode)
-            try:
-                plan = plan_dedisp(self.shifts_host.T, lds_budget=self.lds_budget, **kw)
-            except RuntimeError:
-                # wide plans: one work-group per CU with the whole LDS
-                plan = plan_dedisp(self.shifts_host.T, lds_budget=160 * 1024, **kw)
+            kw = dict(n_widths=self.n_widths, n_centres=n_centres, ring_dtype=self.ring_dtype)
+            # the node's layout within its budget; wide plans: one work-group per CU with
+            # the whole LDS; two window buffers that do not fit even then: one buffer
+            tries = [(self.dedisp_mode, self.lds_budget), (self.dedisp_mode, 160 * 1024)]
+            if self.dedisp_mode == 3:
+                tries += [(2, LDS_BUDGET_DEDISP), (2, 160 * 1024)]
+            plan, err = None, None
+            for mode, budget in tries:
+                try:
+                    plan = plan_dedisp(self.shifts_host.T, lds_budget=budget, mode=mode, **kw)
+                    break
+                except RuntimeError as e:
+                    err = e
+            if plan is None:
+                raise err
             dev = {k: torch.from_numpy(np.ascontiguousarray(getattr(plan, k))).to(self.dev)
                    for k in ("sub_info", "rows", "tbase")}
             self.plans[n_centres] = (plan, dev)
@@ -260,12 +278,14 @@ class SearchNode:
                   float(cfg.min_stat_count), _lib.ptr(cand), self.cand_cap, _lib.ptr(n_cand), _lib.stream(s))
 
     def autotune(self, n_steps: int, *, budgets=(16 << 10, 24 << 10, 32 << 10, 48 << 10, 64 << 10, 80 << 10, 112 << 10, 160 << 10),
-                 modes=(2, 1), reps: int = 2) -> List[dict]:
+                 modes: Optional[tuple] = None, reps: int = 2) -> List[dict]:
         """Time the dedispersion kernel under each LDS layout the planner
         offers (time tile from the LDS budget; unpaired or subband-pair rows)
         on the current history and keep the fastest for blocks of ``n_steps``.
         Runs on scratch copies of the statistics and candidate buffers, so the
         stream's state is untouched.  Returns the timings."""
+        if modes is None:
+            modes = (2, 1, 3) if self.ring_dtype == torch.float32 else (2, 1)
         s = self._stream(None)
         mean, m2 = self.mean.clone(), self.m2.clone()
         cand = torch.empty_like(self.cand_slots[0])
@@ -278,7 +298,7 @@ class SearchNode:
                     plan = plan_dedisp(self.shifts_host.T, lds_budget=b, mode=mode, **kw)
                 except RuntimeError:
                     continue
-                key = (plan.tile, plan.paired)
+                key = (plan.tile, plan.paired, plan.q_total > 0 and not plan.paired)
                 if key in seen:
                     continue
                 seen.add(key)
@@ -297,7 +317,8 @@ class SearchNode:
                         e1.synchronize()
                         ms.append(e0.elapsed_time(e1))
                 t = min(ms)
-                rows.append({"tile": plan.tile, "paired": plan.paired, "lds_bytes": plan.lds_bytes, "ms": t})
+                rows.append({"tile": plan.tile, "paired": plan.paired, "async": bool(plan.q_total > 0 and not plan.paired),
+                             "lds_bytes": plan.lds_bytes, "ms": t})
                 if best is None or t < best[0]:
                     best = (t, plan, pd)
         if best is None:
@@ -382,7 +403,7 @@ class MultiRangeNode:
     trials."""
 
     def __init__(self, cfg: NodeConfig, antpos_e: np.ndarray, antpos_n: np.ndarray, *,
-                 device: str = "cuda", ring_dtype: torch.dtype = torch.float16,
+                 device: str = "cuda", ring_dtype: torch.dtype = torch.float32,
                  max_block_steps: Optional[int] = None, stream: Optional[torch.cuda.Stream] = None):
         if not cfg.dm_ranges:
             raise ValueError("MultiRangeNode needs cfg.dm_ranges")

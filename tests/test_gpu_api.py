@@ -156,12 +156,12 @@ def test_multi_boxcar_argmax():
     assert np.array_equal(win.cpu().numpy().reshape(T - maxw, -1), scores.argmax(axis=0))
 
 
-@pytest.mark.parametrize("dtype,hist", [(torch.float16, 24), (torch.float16, 21), (torch.float32, 24)])
+@pytest.mark.parametrize("dtype,hist", [(torch.float16, 24), (torch.float16, 21), (torch.float32, 24), (torch.float32, 22)])
 def test_ring_advance_matches_numpy(dtype, hist):
     """dsx_ring_advance over three blocks (64, 128, 64 steps): the history
     rows hold the last `hist` steps of the previous contents followed by the
-    new block, time-major per (subband, pixel).  hist = 24 (fp16) takes the
-    16-byte path, hist = 21 the element path; both are exact copies."""
+    new block, time-major per (subband, pixel).  hist = 24 takes the 16-byte
+    paths (fp16 and fp32), hist = 21 / 22 the element paths; all are exact."""
     from dsx import _lib
     n_sub, n_pix, cap = 3, 128, 128
     row_len = hist + cap

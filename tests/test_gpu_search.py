@@ -44,7 +44,7 @@ def _cand_keys(cands):
     return [(c.dm_idx, c.time_bin, c.width_samples, int(c.l), int(c.m)) for c in cands]
 
 
-@pytest.mark.parametrize("layout", ["unpaired", "paired", "autotuned", "fp16", "fp16-autotuned"])
+@pytest.mark.parametrize("layout", ["unpaired", "paired", "async", "autotuned", "fp16", "fp16-autotuned"])
 def test_dedisp_detect_random_images_vs_oracle(layout):
     """The LDS layouts the planner offers (unpaired subband windows, subband
     pair rows, and whichever the autotuner keeps) give the oracle's records,
@@ -58,7 +58,8 @@ def test_dedisp_detect_random_images_vs_oracle(layout):
     print(f"layout {layout}", flush=True)
     fp16 = layout.startswith("fp16")
     dt = torch.float16 if fp16 else torch.float32
-    node = SearchNode(cfg, ant_e, ant_n, ring_dtype=dt, dedisp_mode=1 if layout == "paired" else 2)
+    node = SearchNode(cfg, ant_e, ant_n, ring_dtype=dt,
+                      dedisp_mode={"paired": 1, "async": 3}.get(layout, 2))
     o = OracleStream(cfg, ant_e, ant_n)
     sh = cfg.shift_table()
     L = cfg.n_steps
@@ -79,10 +80,17 @@ def test_dedisp_detect_random_images_vs_oracle(layout):
         if layout.endswith("autotuned") and blk == 1:
             rows = node.autotune(L)
             assert len(rows) >= 2 and {r["paired"] for r in rows} == {False, True}
+            if not fp16:
+                assert any(r["async"] for r in rows)
             best = min(rows, key=lambda r: r["ms"])
-            assert node.plans[L][0].tile == best["tile"] and node.plans[L][0].paired == best["paired"]
+            kept = node.plans[L][0]
+            assert (kept.tile, kept.paired, bool(kept.q_total > 0 and not kept.paired)) == \
+                (best["tile"], best["paired"], best["async"])
         if layout == "paired":
             assert node._plan_for(L)[0].paired
+        if layout == "async":
+            pl = node._plan_for(L)[0]
+            assert not pl.paired and pl.q_total == pl.r_total > 0
         res = node.detect(L)
         o.push_images(img.astype(np.float64))
         g = _gpu_records(res.records)
@@ -96,7 +104,7 @@ def test_dedisp_detect_random_images_vs_oracle(layout):
     assert total > 0
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_dedisp_detect_ragged_shapes_vs_oracle(mode):
     """Ragged shapes: 300 DM trials (two work-groups along DMs, the second
     with 44 live lanes), 7 subbands (the last subband pair has no second
@@ -114,6 +122,7 @@ def test_dedisp_detect_ragged_shapes_vs_oracle(mode):
     sh = cfg.shift_table()
     L, N = cfg.n_steps, cfg.n_grid
     assert node._plan_for(L)[0].paired == (mode == 1)
+    assert (node._plan_for(L)[0].q_total > 0) == (mode != 2)
     total = 0
     for blk in range(5):
         img = rng.standard_normal((cfg.n_sub, L, N * N)).astype(np.float32)
