@@ -86,34 +86,41 @@ __global__ __launch_bounds__(256) void ring_transpose16_kernel(const _Float16 *i
 }
 
 // float32 history with 16-byte accesses (n_pix, n_new multiples of 64; hist, row_len
-// multiples of 4): tile rows read as 4-pixel vectors, each thread writes 4
-// consecutive steps of one pixel.
+// multiples of 4).  A TT-step x 64-pixel tile (TT = 256, 128 or 64, the largest
+// dividing the block) is read as 4-pixel vectors (16 lanes cover a 256-byte
+// image-row segment) and written as 4-step vectors, TT / 4 lanes per pixel
+// (TT x 4 contiguous bytes of its history row).  The tile's 4-float groups
+// are XOR-swizzled by row group, so the column reads of the write phase spread
+// over the LDS banks.
+template <int TT>
 __global__ __launch_bounds__(256) void ring_transpose16f_kernel(const float *img, float *ring, int n_sub,
                                                                 int n_pix, int n_new, int hist, int row_len) {
-    __shared__ __align__(16) float tile[64][68];        // [step][pixel], 272-byte rows
+    __shared__ __align__(16) float tile[TT][64];        // [step][pixel group ^ (step / 4)]
+    constexpr int PER = TT * 16 / 256;                   // 16-byte pieces per thread and phase
     const int tid = threadIdx.x;
-    const int tiles_p = n_pix / 64, tiles_t = n_new / 64;
+    const int tiles_p = n_pix / 64, tiles_t = n_new / TT;
     const int64_t n_tiles = (int64_t)n_sub * tiles_p * tiles_t;
     for (int64_t b = blockIdx.x; b < n_tiles; b += gridDim.x) {
         const int s = (int)(b / ((int64_t)tiles_p * tiles_t));
         const int rem = (int)(b - (int64_t)s * tiles_p * tiles_t);
         const int tt = rem / tiles_p, tp = rem - tt * tiles_p;
-        const int t0 = tt * 64, p0 = tp * 64;
+        const int t0 = tt * TT, p0 = tp * 64;
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < PER; ++k) {
             const int i = tid + 256 * k;
-            const int r = i >> 4, j = i & 15;
-            *reinterpret_cast<float4 *>(&tile[r][4 * j]) =
+            const int r = i >> 4, j = i & 15;                     // step r, pixels 4j .. 4j + 3
+            *reinterpret_cast<float4 *>(&tile[r][4 * (j ^ ((r >> 2) & 15))]) =
                 *reinterpret_cast<const float4 *>(img + ((int64_t)s * n_new + t0 + r) * n_pix + p0 + 4 * j);
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < PER; ++k) {
             const int i = tid + 256 * k;
-            const int q = i & 63, c = i >> 6;
+            const int q = i / (TT / 4), c = i % (TT / 4);         // pixel q, steps 4c .. 4c + 3
+            const int col = 4 * ((q >> 2) ^ (c & 15)) + (q & 3);
             *reinterpret_cast<float4 *>(ring + ((int64_t)s * n_pix + p0 + q) * row_len + hist + t0 + 4 * c) =
-                make_float4(tile[4 * c][q], tile[4 * c + 1][q], tile[4 * c + 2][q], tile[4 * c + 3][q]);
+                make_float4(tile[4 * c][col], tile[4 * c + 1][col], tile[4 * c + 2][col], tile[4 * c + 3][col]);
         }
     }
 }
@@ -199,8 +206,18 @@ extern "C" int dsx_ring_advance(const void *ring_old, void *ring_new, const void
         if (nv > 0)
             ring_hist_vec_kernel<float><<<(int)std::min<int64_t>((nv + 255) / 256, 65536), 256, 0, s>>>(
                 (const float *)ring_old, (float *)ring_new, n_rows, hist, row_len, prev_len);
-        ring_transpose16f_kernel<<<tb, 256, 0, s>>>((const float *)img, (float *)ring_new, n_sub, n_pix, n_new,
-                                                    hist, row_len);
+        if (n_new % 256 == 0) {
+            const int tb3 = (int)std::min<int64_t>((int64_t)n_sub * (n_pix / 64) * (n_new / 256), 65536);
+            ring_transpose16f_kernel<256><<<tb3, 256, 0, s>>>((const float *)img, (float *)ring_new, n_sub, n_pix,
+                                                              n_new, hist, row_len);
+        } else if (n_new % 128 == 0) {
+            const int tb2 = (int)std::min<int64_t>((int64_t)n_sub * (n_pix / 64) * (n_new / 128), 65536);
+            ring_transpose16f_kernel<128><<<tb2, 256, 0, s>>>((const float *)img, (float *)ring_new, n_sub, n_pix,
+                                                              n_new, hist, row_len);
+        } else {
+            ring_transpose16f_kernel<64><<<tb, 256, 0, s>>>((const float *)img, (float *)ring_new, n_sub, n_pix,
+                                                            n_new, hist, row_len);
+        }
     } else if (vec16) {
         const int64_t nv = n_rows * (hist / 8);
         if (nv > 0)

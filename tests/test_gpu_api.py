@@ -158,19 +158,19 @@ def test_multi_boxcar_argmax():
 
 @pytest.mark.parametrize("dtype,hist", [(torch.float16, 24), (torch.float16, 21), (torch.float32, 24), (torch.float32, 22)])
 def test_ring_advance_matches_numpy(dtype, hist):
-    """dsx_ring_advance over three blocks (64, 128, 64 steps): the history
+    """dsx_ring_advance over four blocks (64, 256, 128, 64 steps): the history
     rows hold the last `hist` steps of the previous contents followed by the
     new block, time-major per (subband, pixel).  hist = 24 takes the 16-byte
     paths (fp16 and fp32), hist = 21 / 22 the element paths; all are exact."""
     from dsx import _lib
-    n_sub, n_pix, cap = 3, 128, 128
+    n_sub, n_pix, cap = 3, 128, 256
     row_len = hist + cap
     code = _lib.DSX_F16 if dtype == torch.float16 else _lib.DSX_F32
     rings = [torch.zeros((n_sub, n_pix, row_len), dtype=dtype, device="cuda") for _ in range(2)]
     want = np.zeros((n_sub, n_pix, row_len))
     rng = np.random.default_rng(5)
     cur, prev_len = 0, 0
-    for n_new in (64, 128, 64):
+    for n_new in (64, 256, 128, 64):
         img = rng.standard_normal((n_sub, n_new, n_pix)).astype(np.float32)
         img_t = torch.from_numpy(img).to(dtype).cuda().contiguous()
         img = img_t.float().cpu().numpy()
