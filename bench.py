@@ -326,7 +326,7 @@ def main():
                               "kernel is not HBM-bound: per (DM, step, pixel) 16 float32 window reads from LDS "
                               "and ~28 VALU instructions (hardware counters: 27.9 VALU and 8.7 LDS instructions "
                               "per 64 lanes); with two window buffers filled by LDS-direct loads (fp32 history) "
-                              "VALU issues on 70% of SIMD cycles and LDS on 88% of CU cycles")},
+                              "VALU issues on 70% of SIMD cycles: bound by VALU issue and latency")},
     }
     ops = cfg.n_dm * cfg.n_steps * n_pix * (cfg.n_sub + 3 * len(cfg.widths) + 4)
     out["roofline"]["valu_tops"] = ops / (dd_ms * 1e-3) / 1e12
