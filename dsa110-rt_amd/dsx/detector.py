@@ -54,7 +54,7 @@ class Candidate:
 
     @property
     def key(self) -> Tuple[int, int, int, int, int]:
-        """North-star identity: (DM index, time bin, width, row, col)."""
+        """Candidate identity used to compare candidate lists: (DM index, time bin, width, row, col)."""
         return (self.dm_idx, self.time_bin, self.width_samples, int(self.l), int(self.m))
 
 

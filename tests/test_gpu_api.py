@@ -21,15 +21,15 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def c1():
+def small():
     cfg = dcfg.small_node_config().with_(n_samp=256)
     sc = synth.make_scenario(cfg)
     return cfg, sc, sc.block(0, 256)
 
 
 @pytest.mark.parametrize("dtype", [torch.int8, torch.float16, torch.float32])
-def test_unpack_int4_split(c1, dtype):
-    cfg, sc, raw = c1
+def test_unpack_int4_split(small, dtype):
+    cfg, sc, raw = small
     re, im = unpack_int4_split(raw, device="cuda", scale=0.05, out_dtype=dtype, shape=(128, 16, 64))
     assert re.shape == (64, 2, 2, 128, 16)
     lo, hi = ocorr.unpack_bytes(raw)                      # (pkt, ant, ch, t, p)
@@ -42,15 +42,15 @@ def test_unpack_int4_split(c1, dtype):
         np.testing.assert_allclose(im.float().cpu().numpy(), want_im * np.float32(0.05), rtol=1e-3)
 
 
-def test_unpack_rejects_wrong_size(c1):
-    cfg, sc, raw = c1
+def test_unpack_rejects_wrong_size(small):
+    cfg, sc, raw = small
     with pytest.raises(ValueError):
         unpack_int4_split(raw[:10], shape=(128, 16, 64))
 
 
 @pytest.mark.parametrize("fuse,csf", [(False, 1), (True, 1), (True, 4)])
-def test_fast_corr_compute_split_exact(c1, fuse, csf):
-    cfg, sc, raw = c1
+def test_fast_corr_compute_split_exact(small, fuse, csf):
+    cfg, sc, raw = small
     re, im = unpack_int4_split(raw, out_dtype=torch.int8, shape=(128, 16, 64))
     k = FastCorrKernel(device="cuda", t_int_fast_native=8, nants=16, nchan=64, accum_dtype=torch.int32)
     got = k.compute_split(re, im, fuse_stokes_i=fuse, chan_sum_factor=csf).cpu().numpy()
@@ -62,8 +62,8 @@ def test_fast_corr_compute_split_exact(c1, fuse, csf):
     assert np.array_equal(got_raw, got)
 
 
-def test_fast_corr_float_path_and_stokes(c1):
-    cfg, sc, raw = c1
+def test_fast_corr_float_path_and_stokes(small):
+    cfg, sc, raw = small
     re, im = unpack_int4_split(raw, out_dtype=torch.float32, scale=0.05, shape=(128, 16, 64))
     k = FastCorrKernel(device="cuda", t_int_fast_native=8, nants=16, nchan=64)
     v2 = k.compute_split(re, im)                            # (fv, nbase, nch, 2) complex64
@@ -74,8 +74,8 @@ def test_fast_corr_float_path_and_stokes(c1):
     np.testing.assert_allclose(stokes_i_pol_sum(v2).cpu().numpy(), vi.cpu().numpy(), rtol=1e-6, atol=1e-6)
 
 
-def test_pattern_and_gridder(c1):
-    cfg, sc, raw = c1
+def test_pattern_and_gridder(small):
+    cfg, sc, raw = small
     f = np.asarray(cfg.chan_freqs_GHz)
     pat = build_pattern(sc.antpos_e, sc.antpos_n, chgroup=0, dec_deg=71.6, n_grid=64, chan_freqs_GHz=f)
     g = FastVisGridder.from_pattern(pat, sc.antpos_e, sc.antpos_n, device="cuda")
