@@ -137,7 +137,11 @@ int dsx_downsample_time(const void *in, void *out, int n_planes, int n_in, int64
  * host_shift is [s][d] (host memory).  Picks the largest time tile (multiple
  * of 64 dividing n_centres, <= 2048) whose windows fit lds_budget bytes.
  * mode 0: pair rows when they fit, else unpaired; 1: pair rows only;
- * 2: unpaired only.  Outputs (host memory):
+ * 2: unpaired only; 3: unpaired with two window buffers (float32 history
+ * only): dsx_dedisp_detect copies the next tile's windows from HBM straight
+ * into the second buffer (LDS-direct loads) while the current tile computes;
+ * the plan marks the second buffer with q_total = r_total.  Outputs (host
+ * memory):
  *   sub_info [n_sub][3] = (window offset, max_d shift[s][d], window length)
  *   rows     [rows_cap][6] pair rows (Q offset, length, R index of subband a
  *            at e = 0, R index of subband b at e = 0, b's window [lo, hi))
@@ -169,7 +173,8 @@ int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, int n_widths
  * into the float64 running mean / m2 ([p][d] layout) with blk_count[d]
  * samples.  sub_info, rows, tbase, n_rows, n_terms, tile, r_total and
  * q_total come from dsx_dedisp_plan (copied to device memory; rows may be
- * NULL when n_rows == 0).  ring_dtype DSX_F16 or DSX_F32;
+ * NULL when n_rows == 0; an unpaired plan with q_total == r_total selects the
+ * two-buffer layout of mode 3 and needs ring_dtype DSX_F32).  ring_dtype DSX_F16 or DSX_F32;
  * hist >= 64 + w_max/2 + max shift.
  * Replaces combine_chgroups + boxcar_via_cumsum + multi_boxcar_argmax +
  * the Layer-1/Layer-2 normalisation, src/dsart/fine_dm/combiner.py:548-655,
