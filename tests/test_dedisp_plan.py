@@ -50,7 +50,8 @@ def _model(plan, sh_sd, n_widths, tt, rng):
 
 
 @pytest.mark.parametrize("name,mode,dtype", [("96ant", 0, torch.float16), ("96ant", 2, torch.float16),
-                                              ("96ant", 1, torch.float32), ("16ant", 0, torch.float16)])
+                                              ("96ant", 1, torch.float32), ("96ant", 3, torch.float32),
+                                              ("16ant", 0, torch.float16)])
 def test_plan_reproduces_direct_dedispersion(name, mode, dtype):
     cfg = dcfg.search_node_config() if name == "96ant" else dcfg.small_node_config()
     sh_sd = np.ascontiguousarray(cfg.shift_table().T)
@@ -62,7 +63,10 @@ def test_plan_reproduces_direct_dedispersion(name, mode, dtype):
     if mode == 1:
         assert plan.paired
     if mode == 2:
-        assert not plan.paired and plan.n_terms == cfg.n_sub
+        assert not plan.paired and plan.n_terms == cfg.n_sub and plan.q_total == 0
+    if mode == 3:     # two window buffers of the same layout; the second sits where pair rows would
+        assert not plan.paired and plan.n_terms == cfg.n_sub and plan.q_total == plan.r_total
+        assert plan.lds_bytes >= 2 * 4 * plan.r_total
     D, want = _model(plan, sh_sd, nw, plan.tile, np.random.default_rng(3))
     np.testing.assert_allclose(D, want, rtol=1e-12, atol=1e-12)
 
@@ -90,3 +94,5 @@ def test_plan_errors():
         plan_dedisp(sh, n_widths=3, n_centres=100, ring_dtype=torch.float16, lds_budget=65536)
     with pytest.raises(RuntimeError, match="LDS"):
         plan_dedisp(sh, n_widths=3, n_centres=64, ring_dtype=torch.float16, lds_budget=100)
+    with pytest.raises(RuntimeError, match="float32 history"):
+        plan_dedisp(sh, n_widths=3, n_centres=64, ring_dtype=torch.float16, lds_budget=65536, mode=3)
