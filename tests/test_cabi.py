@@ -51,6 +51,16 @@ def test_version_and_error_paths():
     assert rc == -1 and b"power of two" in L.dsx_last_error()
     rc = L.dsx_ring_advance(p, p, p, 1, 1, 1, 1, 0, 1, 1, None)
     assert rc == -1 and b"differ" in L.dsx_last_error()
+    # two window buffers (q_total == r_total on an unpaired plan) need a float32 history (ring_dtype 1 = fp16)
+    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, p, p, 0, 64, 1, 64, 10, 10,
+                             ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
+    assert rc == -1 and b"describe a plan" in L.dsx_last_error()
+    buf2 = ctypes.create_string_buffer(64)
+    p2 = ctypes.cast(buf2, ctypes.c_void_p)
+    rc = L.dsx_downsample_time(p, p2, 1, 10, 4, 3, 0, None)
+    assert rc == -1 and b"must divide" in L.dsx_last_error()
+    rc = L.dsx_combine(p, p, p2, 0, 4, 4, 0, 4, None)
+    assert rc == -1 and b"bad shape" in L.dsx_last_error()
 
 
 def test_call_raises_runtime_error():
