@@ -30,6 +30,7 @@ import torch
 
 HBM_PEAK_GBPS = 8000.0
 LDS_PEAK_TBPS = 256 * 128 * 2.4e9 / 1e12     # MI355X LDS read bandwidth (128 B/clk/CU)
+INT8_PEAK_POPS = 5.0                           # MI355X dense int8 MFMA
 MFMA_I8_PEAK_TOPS = 5000.0
 FP32_PEAK_TFLOPS = 157.3
 
@@ -336,6 +337,24 @@ def main():
     lds_tbps = lds_bytes / (dd_ms * 1e-3) / 1e12
     out["roofline"]["lds"] = {"bytes_per_launch": lds_bytes, "achieved_TBps": lds_tbps,
                               "peak_TBps": LDS_PEAK_TBPS, "frac": lds_tbps / LDS_PEAK_TBPS}
+    # every stage against the gfx950 peaks: algorithmic HBM bytes per block / HIP-event stage time,
+    # and the correlator's int8 MFMA ops / time against 5.0 POP/s dense
+    esz = 2 if args.history == "fp16" else 4
+    vis_bytes = cfg.n_steps * cfg.n_chan_out * cfg.nbase * 2 * 4
+    raw_bytes = cfg.n_pkt * cfg.n_ant * cfg.n_chan * 4
+    img_bytes = cfg.n_sub * cfg.n_steps * n_pix * esz
+    hist_bytes = 2 * cfg.n_sub * n_pix * node.hist * esz
+    stage_bytes = {"corr": raw_bytes + vis_bytes, "image": vis_bytes + img_bytes,
+                   "ring": 2 * img_bytes + hist_bytes, "dedisp": alg_bytes}
+    stages = {}
+    for k, b in stage_bytes.items():
+        ms = chosen["stage_ms"][k]
+        gbps = b / (ms * 1e-3) / 1e9
+        stages[k] = {"ms": ms, "alg_bytes": b, "GBps": gbps, "hbm_frac": gbps / HBM_PEAK_GBPS}
+    corr_ops = 8 * cfg.nbase * cfg.n_chan * 2 * cfg.n_samp
+    stages["corr"]["int8_pops"] = corr_ops / (chosen["stage_ms"]["corr"] * 1e-3) / 1e15
+    stages["corr"]["int8_mfma_frac"] = stages["corr"]["int8_pops"] / INT8_PEAK_POPS
+    out["stage_roofline"] = stages
     del node
     if not args.no_plain:
         torch.cuda.empty_cache()
