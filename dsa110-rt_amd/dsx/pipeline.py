@@ -400,11 +400,14 @@ class MultiRangeNode:
     widths in samples of that resolution, float64 statistics and LDS plan.
     Candidates of all ranges are decoded together in full-resolution steps:
     time ``t * f``, width ``w * f``, DM index offset by the earlier ranges'
-    trials."""
+    trials.  ``cand_cap`` bounds each range's chunk-maximum records per block;
+    the block's ``overflow`` is set when any range exceeded it and ``n_raw``
+    counts every range's records, kept or not."""
 
     def __init__(self, cfg: NodeConfig, antpos_e: np.ndarray, antpos_n: np.ndarray, *,
                  device: str = "cuda", ring_dtype: torch.dtype = torch.float32,
-                 max_block_steps: Optional[int] = None, stream: Optional[torch.cuda.Stream] = None):
+                 max_block_steps: Optional[int] = None, stream: Optional[torch.cuda.Stream] = None,
+                 cand_cap: int = 1 << 20):
         if not cfg.dm_ranges:
             raise ValueError("MultiRangeNode needs cfg.dm_ranges")
         cfg.validate()
@@ -417,7 +420,8 @@ class MultiRangeNode:
         off = 0
         for rc, (lo, hi, n, f, _) in zip(cfg.range_configs(), cfg.dm_ranges):
             nd = SearchNode(rc, antpos_e, antpos_n, device=device, ring_dtype=ring_dtype,
-                            max_block_steps=cap // int(f), stream=stream, with_front=False)
+                            max_block_steps=cap // int(f), stream=stream, with_front=False,
+                            cand_cap=cand_cap)
             buf = None if int(f) == 1 else torch.empty(cfg.n_sub * (cap // int(f)) * cfg.n_grid ** 2,
                                                        dtype=ring_dtype, device=torch.device(device))
             self.ranges.append((nd, int(f), off, buf))
@@ -453,9 +457,12 @@ class MultiRangeNode:
             nd.push_images(img_f, n_steps // f, s)
             pends.append(nd.detect_async(n_steps // f, 0, s))
         recs, T0 = [], None
-        k_off = 0
+        k_off = n_raw = 0
+        overflow = False
         for (nd, f, off, _), pend in zip(self.ranges, pends):
             res = nd.finish(pend)
+            n_raw += res.n_raw
+            overflow = overflow or res.overflow
             T0 = pend.T0 * f if T0 is None else T0
             r = res.records.copy()
             r["t"] = r["t"] * f
@@ -467,5 +474,5 @@ class MultiRangeNode:
         cands = decode_chunk_maxima(rec, widths=self.widths_all, n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
                                     threshold=cfg.threshold, n_top=cfg.n_top, radius_t=cfg.radius_t,
                                     radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm, t_int=cfg.t_int)
-        return BlockResult(candidates=cands, n_raw=int(rec.size), overflow=False, T0=int(T0), n_steps=n_steps,
+        return BlockResult(candidates=cands, n_raw=int(n_raw), overflow=overflow, T0=int(T0), n_steps=n_steps,
                            records=rec)
