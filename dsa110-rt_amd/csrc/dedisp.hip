@@ -14,11 +14,21 @@
 // subbands with few DMs each) run unpaired: the lanes sum the subband
 // windows directly.
 //
+// Centring.  Each lane subtracts a bias b from every dedispersed sample,
+// D' = D - b, where b is its float64 running mean rounded to 11 significant
+// bits (or, in a lane's first block, the mean of up to 64 complete samples of
+// this block read from the history, rounded the same way).  Prefix sums, boxcars and the block's statistics are
+// accumulated on D', so their float32 magnitudes stay at the noise level even
+// for pixels holding a steady source; the float64 merge and the S/N add b
+// back exactly.  Lanes not yet scored sum their block statistics directly
+// rather than from prefix differences, since a rough first-block bias lets
+// their prefix drift.
+//
 // Each lane sweeps time in 64-centre chunks and keeps the exclusive prefix
-// sum P of its dedispersed series in a register window
+// sum P of its centred series in a register window
 //     Pw[x] = P[c - W/2 + x],  x in [0, 64 + 2 (W/2)]
-// so every D(t) is formed once; width-1 maxima and the block's statistics
-// come from the fresh D values, the other widths from prefix differences
+// so every D'(t) is formed once; width-1 maxima and the block's statistics
+// come from the fresh D' values, the other widths from prefix differences
 //     B_w(c + i) = Pw[i + W/2 - w/2 + w] - Pw[i + W/2 - w/2].
 // The chunk maximum of each width becomes an S/N with the float64 state read
 // at kernel start; a chunk whose best S/N exceeds the threshold appends one
@@ -36,7 +46,7 @@
 namespace dsx {
 
 struct Cand {
-    int32_t t;
+    int32_t t;      // centre step relative to c_lo (the caller adds the int64 c_lo)
     int32_t p;
     int16_t d;
     int16_t k;
@@ -46,6 +56,7 @@ static_assert(sizeof(Cand) == 16, "candidate record must be 16 bytes");
 
 struct DedispParams {
     const void *ring;
+    const int32_t *shift;       // [s][d] steps (a lane's first-block bias sample)
     const int32_t *sub_info;    // [s][3] = (R offset, shmax_s, window length L_s)
     const int32_t *rows;        // [r][6] = (Q offset, length, R index of a at e = 0,
                                 //           R index of b at e = 0, b's R bounds [lo, hi); empty
@@ -54,10 +65,12 @@ struct DedispParams {
     const int32_t *valid_from;  // [d]
     const double *n_prev;       // [d]
     const double *blk_count;    // [d]
-    double *mean;               // [p][d]
-    double *m2;                 // [p][d]
+    const double *mean;         // [p][d] state of earlier blocks
+    const double *m2;           // [p][d]
+    double *mean_out;           // [p][d] merged state (may alias mean; NULL: no merge)
+    double *m2_out;
     Cand *cand;
-    int32_t *n_cand;
+    int32_t *n_cand;            // records appended (kept or not)
     int64_t T0, c_lo;
     int n_sub, n_pix, hist, row_len, n_dm, n_rows, n_terms, n_centres, tt, r_total, q_total,
         cand_cap;
@@ -65,35 +78,49 @@ struct DedispParams {
     double min_count;
 };
 
+// x rounded to 11 significant bits (fp16 precision, float32 range).  Centring
+// on such a value adds no bits finer than the samples' own at the mean's
+// magnitude, so sums of fp16-valued samples that were exact in float32 stay
+// exact, and exact ties between boxes resolve as in exact arithmetic.
+__device__ __forceinline__ float round_sig11(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7F800000u) == 0x7F800000u) return x;      // inf / nan
+    u = (u + 0x1000u) & ~0x1FFFu;
+    return __uint_as_float(u);
+}
+
 constexpr int DD_CHUNK = 64;      // centres per candidate chunk
 constexpr int DD_SUB = 32;        // centres per register pass (two per chunk)
 constexpr int DD_THREADS = 256;   // DMs per work-group (grid.y covers the rest)
 constexpr int DD_MAX_ROWS = 4096;
 constexpr int DD_STAGE_ITERS = 12;   // fast path: a subband window spans at most 12 x 64 steps
 constexpr int DD_STAGE_SUBS = 4;     // fast path: subbands a wave stages per batch of loads
+constexpr int DD_BIAS_SAMPLES = 64;  // samples behind a lane's first-block centring bias
 
-// D(t) of DM d from the term rows, in the main path's order;
-// t_rel = t - (first centre of the tile).
+// Centred D'(t) = D(t) - bias of DM d from the term rows, in the main path's
+// order; t_rel = t - (first centre of the tile).
 template <typename E>
 __device__ __forceinline__ float terms_d(const E *X, const int32_t *tbase, int n_terms, int n_dm, int d,
-                                         int t_rel) {
-    float acc = (float)X[tbase[d] + t_rel];
+                                         int t_rel, float bias) {
+    float acc = (float)X[tbase[d] + t_rel] - bias;
     for (int g = 1; g < n_terms; ++g) acc += (float)X[tbase[g * n_dm + d] + t_rel];
     return acc;
 }
 
-// Pw[OFF + j] (=|+=) sum over terms g of X[base_g + rel + j], j < N.  With
-// MAXT > 0 the term bases sit in this lane's LDS slots (tb[h * stride], two
-// 16-bit bases per word) and the term loop is unrolled; otherwise they are
-// read from global memory term by term.
+// Pw[OFF + j] (=|+=) (sum over terms g of X[base_g + rel + j]) - bias, j < N,
+// summed as ((x_0 - bias) + x_1) + ...  With MAXT > 0 the term bases sit in
+// this lane's LDS slots (tb[h * stride], two 16-bit bases per word) and the
+// term loop is unrolled; otherwise they are read from global memory term by
+// term.
 template <int MAXT, int N, int OFF, typename E, int NP>
 __device__ __forceinline__ void sum_terms(float (&Pw)[NP], const E *X, const uint32_t *tb, int stride,
-                                          const int32_t *tbase, int n_terms, int n_dm, int d, int rel) {
+                                          const int32_t *tbase, int n_terms, int n_dm, int d, int rel,
+                                          float bias) {
     if constexpr (MAXT > 0) {
         {
             const E *x0 = X + (int)(tb[0] & 0xFFFF) + rel;
 #pragma unroll
-            for (int j = 0; j < N; ++j) Pw[OFF + j] = (float)x0[j];
+            for (int j = 0; j < N; ++j) Pw[OFF + j] = (float)x0[j] - bias;
         }
 #pragma unroll
         for (int g = 1; g < MAXT; ++g) {
@@ -107,7 +134,7 @@ __device__ __forceinline__ void sum_terms(float (&Pw)[NP], const E *X, const uin
         {
             const E *x0 = X + tbase[d] + rel;
 #pragma unroll
-            for (int j = 0; j < N; ++j) Pw[OFF + j] = (float)x0[j];
+            for (int j = 0; j < N; ++j) Pw[OFF + j] = (float)x0[j] - bias;
         }
         for (int g = 1; g < n_terms; ++g) {
             const E *xg = X + tbase[g * n_dm + d] + rel;
@@ -117,8 +144,11 @@ __device__ __forceinline__ void sum_terms(float (&Pw)[NP], const E *X, const uin
     }
 }
 
+// Two-buffer instances with term bases in LDS and widths up to 32 fit 168
+// VGPRs without spills: ask for three waves per SIMD, which is what hides the
+// LDS latency of the term loop.
 template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false>
-__global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
+__global__ __launch_bounds__(DD_THREADS, ASYNC && MAXT > 0 && NW <= 6 ? 3 : 2) void dedisp_kernel(DedispParams P) {
     constexpr int WMAX = 1 << (NW - 1);
     constexpr int H2 = WMAX / 2;
     constexpr int WW = 2 * H2;
@@ -195,10 +225,12 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
 
     for (int p = blockIdx.x; p < P.n_pix; p += gridDim.x) {
         // ---- per-lane state from earlier blocks (float64)
-        // w_k mean and 1 / (sigma sqrt(w_k)) are power-of-two scalings of the
-        // k = 0 / k = 1 values (exact in binary floating point), so three floats
-        // carry them: wmu_k = 2^k wmu0, inv_k = 2^-(k/2) inv_{k mod 2}
-        float wmu0 = 0.f, inv0 = 0.f, inv1 = 0.f;
+        // Samples are centred by `bias` (float32).  A box of width w_k then
+        // scores snr_k = (B'_k + w_k (bias - mean)) / (sigma sqrt(w_k)); w_k times
+        // the residual and 1 / (sigma sqrt(w_k)) are power-of-two scalings of
+        // the k = 0 / k = 1 values (exact in binary floating point), so three
+        // floats carry them: rmu_k = 2^k rmu0, inv_k = 2^-(k/2) inv_{k mod 2}
+        float bias = 0.f, rmu0 = 0.f, inv0 = 0.f, inv1 = 0.f;
         bool score = false;
         int vf = 0;
         if (active) {
@@ -208,7 +240,25 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
             const double m2_a = P.m2[(int64_t)p * P.n_dm + d];
             const double sig = n_a > 0.0 ? sqrt(m2_a / n_a) : 0.0;
             score = (n_a >= P.min_count) && (n_a > 0.0) && (sig > 0.0);
-            wmu0 = (float)mean_a;
+            if (n_a > 0.0) {
+                bias = round_sig11((float)mean_a);
+            } else {
+                // no state yet: centre on the mean of this block's first (up to)
+                // DD_BIAS_SAMPLES complete samples, summed straight from the
+                // history (in bounds: c_lo <= t < c_hi and hist >= 64 + max shift)
+                const int64_t t_b = vf > P.c_lo ? (int64_t)vf : P.c_lo;
+                const int nk = (int)min((int64_t)DD_BIAS_SAMPLES, c_hi - t_b);
+                if (nk > 0) {
+                    float acc = 0.f;
+                    for (int s = 0; s < P.n_sub; ++s) {
+                        const T *row = ring + ((int64_t)s * P.n_pix + p) * P.row_len +
+                                       (P.hist + t_b - P.shift[(int64_t)s * P.n_dm + d] - P.T0);
+                        for (int k = 0; k < nk; ++k) acc += (float)row[k];
+                    }
+                    bias = round_sig11(acc / (float)nk);
+                }
+            }
+            rmu0 = (float)((double)bias - mean_a);
             inv0 = score ? (float)(1.0 / sig) : 0.f;
             inv1 = score ? (float)(1.0 / (sig * sqrt(2.0))) : 0.f;
         }
@@ -319,7 +369,7 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
 
             if (c0 == 0) {
                 // prologue: D(c_lo - H2 + x), x < WW, into Pw[x + 1]
-                if (WW > 0) sum_terms<MAXT, WW, 1>(Pw, X, tb, nthr, P.tbase, P.n_terms, P.n_dm, d, -H2);
+                if (WW > 0) sum_terms<MAXT, WW, 1>(Pw, X, tb, nthr, P.tbase, P.n_terms, P.n_dm, d, -H2, bias);
                 // samples t = c_lo + (x - H2) for x >= H2 belong to this block
                 float a1 = 0.f, a2 = 0.f, m = -INFINITY;
 #pragma unroll
@@ -352,14 +402,26 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                 for (int j = 0; j < DD_CHUNK / DD_SUB; ++j) {
                     const int cs = cc + j * DD_SUB;                  // sub-chunk start (tile relative)
                     // ---- D(c + j S + H2 + i), i < S, summed over the term rows
-                    sum_terms<MAXT, DD_SUB, WW + 1>(Pw, X, tb, nthr, P.tbase, P.n_terms, P.n_dm, d, cs + H2);
+                    sum_terms<MAXT, DD_SUB, WW + 1>(Pw, X, tb, nthr, P.tbase, P.n_terms, P.n_dm, d, cs + H2,
+                                                    bias);
                     // ---- statistics of this block's complete samples
                     const int64_t t1 = c + j * DD_SUB + H2;
                     float a1 = 0.f, a2 = 0.f;
                     const bool full = t1 >= vf && t1 + DD_SUB <= c_hi;
                     if (full) {
+                        if (score) {
 #pragma unroll
-                        for (int i = 0; i < DD_SUB; ++i) a2 = fmaf(Pw[WW + 1 + i], Pw[WW + 1 + i], a2);
+                            for (int i = 0; i < DD_SUB; ++i) a2 = fmaf(Pw[WW + 1 + i], Pw[WW + 1 + i], a2);
+                        } else {
+                            // lanes not yet scored (statistics burn-in) may be centred
+                            // on a rough bias, so their prefix drifts: sum the samples
+                            // directly instead of differencing the prefix
+#pragma unroll
+                            for (int i = 0; i < DD_SUB; ++i) {
+                                a1 += Pw[WW + 1 + i];
+                                a2 = fmaf(Pw[WW + 1 + i], Pw[WW + 1 + i], a2);
+                            }
+                        }
                     } else {
 #pragma unroll
                         for (int i = 0; i < DD_SUB; ++i) {
@@ -379,7 +441,7 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                     // ---- prefix sums in place
 #pragma unroll
                     for (int i = 0; i < DD_SUB; ++i) Pw[WW + 1 + i] += Pw[WW + i];
-                    if (full) a1 = Pw[WW + DD_SUB] - Pw[WW];
+                    if (full && score) a1 = Pw[WW + DD_SUB] - Pw[WW];
                     s1 += (double)a1;
                     s2 += (double)a2;
                     // ---- boxcar maxima of the wider widths over this sub-chunk's centres
@@ -406,7 +468,7 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                     int bk = 0;
 #pragma unroll
                     for (int k = 0; k < NW; ++k) {
-                        const float snr = (mb[k] - ldexpf(wmu0, k)) * ldexpf((k & 1) ? inv1 : inv0, -(k >> 1));
+                        const float snr = (mb[k] + ldexpf(rmu0, k)) * ldexpf((k & 1) ? inv1 : inv0, -(k >> 1));
                         if (snr > best) {
                             best = snr;
                             bk = k;
@@ -420,7 +482,7 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                         int bi = 0;
                         if (bk == 0) {
                             for (int i = 0; i < DD_CHUNK; ++i) {
-                                if (terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc + i) == target) {
+                                if (terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc + i, bias) == target) {
                                     bi = i;
                                     break;
                                 }
@@ -432,7 +494,7 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                             // position x is P[c - H2 + x] = pbase + sum_{x' < x} D(c - H2 + x')
                             float pa = pbase, pb = pbase;
                             for (int x = 0; x < lo + w; ++x) {
-                                const float dv = terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc - H2 + x);
+                                const float dv = terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc - H2 + x, bias);
                                 if (x < lo) pa = dv + pa;
                                 pb = dv + pb;
                             }
@@ -441,14 +503,14 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                                     bi = i;
                                     break;
                                 }
-                                pa = terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc - H2 + lo + i) + pa;
-                                pb = terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc - H2 + lo + w + i) + pb;
+                                pa = terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc - H2 + lo + i, bias) + pa;
+                                pb = terms_d(X, P.tbase, P.n_terms, P.n_dm, d, cc - H2 + lo + w + i, bias) + pb;
                             }
                         }
                         const int slot = atomicAdd(P.n_cand, 1);
                         if (slot < P.cand_cap) {
                             Cand cd;
-                            cd.t = (int32_t)(c + bi);
+                            cd.t = (int32_t)(c + bi - P.c_lo);
                             cd.p = p;
                             cd.d = (int16_t)d;
                             cd.k = (int16_t)bk;
@@ -460,19 +522,23 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
             }
         }
         // ---- merge this block's statistics into the running state
-        if (active) {
+        // (s1, s2 are sums of the centred samples D - bias)
+        if (active && P.mean_out != nullptr) {
             const double nb = P.blk_count[d];
+            const int64_t i = (int64_t)p * P.n_dm + d;
+            // earlier blocks' state, read again rather than kept in registers
+            const double mean_a = P.mean[i];
+            const double m2_a = P.m2[i];
             if (nb > 0.0) {
-                // earlier blocks' state, read again rather than kept in registers
                 const double n_a = P.n_prev[d];
-                const double mean_a = P.mean[(int64_t)p * P.n_dm + d];
-                const double m2_a = P.m2[(int64_t)p * P.n_dm + d];
-                const double mean_b = s1 / nb;
                 const double m2_b = s2 - s1 * s1 / nb;
                 const double n = n_a + nb;
-                const double delta = mean_b - mean_a;
-                P.mean[(int64_t)p * P.n_dm + d] = mean_a + delta * (nb / n);
-                P.m2[(int64_t)p * P.n_dm + d] = m2_a + m2_b + delta * delta * (n_a * nb / n);
+                const double delta = ((double)bias - mean_a) + s1 / nb;     // mean_b - mean_a
+                P.mean_out[i] = mean_a + delta * (nb / n);
+                P.m2_out[i] = m2_a + m2_b + delta * delta * (n_a * nb / n);
+            } else if (P.mean_out != P.mean) {
+                P.mean_out[i] = mean_a;
+                P.m2_out[i] = m2_a;
             }
         }
     }
@@ -707,16 +773,19 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
 }
 
 extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, int n_pix, int hist,
-                                 int ring_len, int64_t T0, const int32_t *sub_info,
+                                 int ring_len, int64_t T0, const int32_t *shift, const int32_t *sub_info,
                                  const int32_t *rows, int n_rows, const int32_t *tbase, int n_terms,
                                  int n_dm, const int32_t *valid_from, const double *n_prev,
-                                 const double *blk_count, double *mean, double *m2, int64_t c_lo,
-                                 int n_centres, int n_widths, int tile, int r_total, int q_total,
-                                 float threshold, double min_count, void *cand, int cand_cap,
-                                 int32_t *n_cand, dsx_stream_t stream) {
-    DSX_REQUIRE(ring && sub_info && tbase && valid_from && n_prev && blk_count && mean && m2 && cand &&
-                    n_cand,
+                                 const double *blk_count, const double *mean, const double *m2,
+                                 double *mean_out, double *m2_out, int64_t c_lo, int n_centres,
+                                 int n_widths, int tile, int r_total, int q_total, float threshold,
+                                 double min_count, void *cand, int cand_cap, int32_t *n_cand,
+                                 dsx_stream_t stream) {
+    DSX_REQUIRE(ring && shift && sub_info && tbase && valid_from && n_prev && blk_count && mean && m2 &&
+                    cand && n_cand,
                 "dsx_dedisp_detect: null pointer");
+    DSX_REQUIRE((mean_out == nullptr) == (m2_out == nullptr),
+                "dsx_dedisp_detect: mean_out and m2_out must both be set or both be NULL");
     DSX_REQUIRE(ring_dtype == DSX_F16 || ring_dtype == DSX_F32, "dsx_dedisp_detect: ring_dtype");
     DSX_REQUIRE(n_sub > 0 && n_pix > 0 && n_dm > 0 && hist >= 0 && ring_len > hist,
                 "dsx_dedisp_detect: bad shape");
@@ -740,6 +809,7 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
         return set_error(DSX_ERANGE, "dsx_dedisp_detect: %zu B of LDS for the time windows", lds);
     DedispParams P{};
     P.ring = ring;
+    P.shift = shift;
     P.sub_info = sub_info;
     P.rows = rows;
     P.tbase = tbase;
@@ -748,6 +818,8 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
     P.blk_count = blk_count;
     P.mean = mean;
     P.m2 = m2;
+    P.mean_out = mean_out;
+    P.m2_out = m2_out;
     P.cand = reinterpret_cast<Cand *>(cand);
     P.n_cand = n_cand;
     P.T0 = T0;

@@ -58,7 +58,31 @@ class Candidate:
         return (self.dm_idx, self.time_bin, self.width_samples, int(self.l), int(self.m))
 
 
+#: 16-byte device record of ``dsx_dedisp_detect``: time relative to the block's first centre.
 CAND_DTYPE = np.dtype([("t", "<i4"), ("p", "<i4"), ("d", "<i2"), ("k", "<i2"), ("snr", "<f4")])
+#: host record: absolute dedispersed step (int64, no wrap however long the stream runs).
+REC_DTYPE = np.dtype([("t", "<i8"), ("p", "<i4"), ("d", "<i2"), ("k", "<i2"), ("snr", "<f4")])
+
+#: thresholds an overflowed block may be detected again at
+N_LEVELS = 16
+
+
+def record_levels(threshold: float) -> np.ndarray:
+    """Overflow ladder: ``threshold * sqrt(2)**j``, ``j < 16``, in float32
+    (the kernel compares float32 S/N against a float32 threshold)."""
+    thr = np.float32(threshold)
+    odd = np.float32(thr * np.float32(1.41421356))
+    return np.array([np.ldexp(odd if j & 1 else thr, j >> 1) for j in range(N_LEVELS)], dtype=np.float32)
+
+
+def device_records(rec_dev: np.ndarray, c_lo: int) -> np.ndarray:
+    """Device records (``CAND_DTYPE``, time relative to ``c_lo``) -> host
+    records (``REC_DTYPE``, absolute int64 time)."""
+    out = np.empty(rec_dev.size, dtype=REC_DTYPE)
+    out["t"] = rec_dev["t"].astype(np.int64) + int(c_lo)
+    for f in ("p", "d", "k", "snr"):
+        out[f] = rec_dev[f]
+    return out
 
 
 def decode_chunk_maxima(rec: np.ndarray, *, widths: Sequence[int], n_grid: int,

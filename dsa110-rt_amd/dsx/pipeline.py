@@ -25,7 +25,8 @@ import torch
 from . import _lib
 from .config import DETECT_CHUNK_STEPS, DETECT_DELAY_STEPS, NodeConfig
 from .dedisp import plan_dedisp
-from .detector import CAND_DTYPE, Candidate, decode_chunk_maxima
+from .detector import (CAND_DTYPE, N_LEVELS, REC_DTYPE, Candidate, decode_chunk_maxima,
+                       device_records, record_levels)
 from .imager import mask_npad
 from .pattern import build_cell_map, build_imager_taps
 
@@ -41,7 +42,12 @@ class BlockResult:
     overflow: bool
     T0: int
     n_steps: int
-    records: Optional[np.ndarray] = None   # raw chunk-maximum records (CAND_DTYPE)
+    records: Optional[np.ndarray] = None   # chunk-maximum records (REC_DTYPE, absolute int64 time)
+    #: threshold the kept records are complete above: the node's threshold, or
+    #: after an overflow the lowest record-count level whose records fit
+    threshold_used: Optional[float] = None
+    #: per DM range of an uneven plan: the threshold its records are complete above
+    range_thresholds: Optional[List[float]] = None
 
 
 class SearchNode:
@@ -107,16 +113,29 @@ class SearchNode:
                       for _ in range(nd)]
         self.cur = 0
         ns = self.n_pix if with_detect else 0
-        self.mean = torch.zeros((ns, cfg.n_dm), dtype=torch.float64, device=self.dev)
-        self.m2 = torch.zeros((ns, cfg.n_dm), dtype=torch.float64, device=self.dev)
+        # float64 noise state in n_slots + 1 rotating buffers: block k reads
+        # buffer k and writes the merge into buffer k + 1, so the state a block
+        # was detected against stays intact until block k + n_slots + 1 -- long
+        # enough to re-detect a block whose records overflowed (finish())
+        self.n_stat = self.n_slots + 1
+        self.mean_bufs = [torch.zeros((ns, cfg.n_dm), dtype=torch.float64, device=self.dev)
+                          for _ in range(self.n_stat)]
+        self.m2_bufs = [torch.zeros((ns, cfg.n_dm), dtype=torch.float64, device=self.dev)
+                        for _ in range(self.n_stat)]
+        self.n_prev_bufs = [torch.zeros(cfg.n_dm, dtype=torch.float64, device=self.dev)
+                            for _ in range(self.n_stat)]
+        self.stat_cur = 0
         self.n_prev_host = np.zeros(cfg.n_dm, dtype=np.float64)
-        self.n_prev = torch.zeros(cfg.n_dm, dtype=torch.float64, device=self.dev)
         self.blk_count = torch.zeros(cfg.n_dm, dtype=torch.float64, device=self.dev)
         self.cand_cap = int(cand_cap)
         self.cand_slots = [torch.empty((self.cand_cap if with_detect else 0, 4), dtype=torch.int32,
                                        device=self.dev) for _ in range(self.n_slots)]
         self.n_cand_slots = [torch.zeros(1, dtype=torch.int32, device=self.dev) for _ in range(self.n_slots)]
         self.cand, self.n_cand = self.cand_slots[0], self.n_cand_slots[0]
+        # which block last wrote each history / state buffer (re-detection check)
+        self.block_seq = 0
+        self.ring_writer = [-1, -1]
+        self.stat_writer = [-1] * self.n_stat
         # pinned host staging per slot: block counts in, candidate count + records out
         self._rec_prefix = min(self.cand_cap, 1 << 16)
         self._blk_pinned = [torch.zeros(cfg.n_dm, dtype=torch.float64, pin_memory=True) for _ in range(self.n_slots)]
@@ -137,6 +156,27 @@ class SearchNode:
         self.T0 = 0
         self.prev_len = 0
         self.marks: Optional[Dict[str, list]] = None     # stage -> [start, end] events when timing
+
+    @property
+    def mean(self) -> torch.Tensor:
+        """float64 running mean ``[pixel, DM]`` the next block is detected against."""
+        return self.mean_bufs[self.stat_cur]
+
+    @property
+    def m2(self) -> torch.Tensor:
+        """float64 running sum of squared deviations ``[pixel, DM]``."""
+        return self.m2_bufs[self.stat_cur]
+
+    @property
+    def n_prev(self) -> torch.Tensor:
+        """Complete samples per DM behind :attr:`mean` / :attr:`m2` (device float64)."""
+        return self.n_prev_bufs[self.stat_cur]
+
+    def device_bytes(self) -> int:
+        """Device bytes this node holds (histories, block slots, state, records)."""
+        ts = [*self.rings, *self.vis_slots, *self.img_slots, *self.cand_slots, *self.mean_bufs,
+              *self.m2_bufs]
+        return int(sum(t.numel() * t.element_size() for t in ts))
 
     def enable_timing(self) -> None:
         """Record HIP events around every kernel (read with :meth:`stage_ms`)."""
@@ -229,6 +269,7 @@ class SearchNode:
         if img.numel() < cfg.n_sub * n_steps * self.n_pix or not img.is_contiguous():
             raise ValueError(f"images must be contiguous [n_sub, >= {n_steps}, n_grid, n_grid]")
         old, new = self.rings[self.cur], self.rings[1 - self.cur]
+        self.ring_writer[1 - self.cur] = self.block_seq
         self._mark("ring", 0)
         _lib.call("dsx_ring_advance", _lib.ptr(old), _lib.ptr(new), _lib.ptr(img), cfg.n_sub,
                   self.n_pix, self.hist, self.cap, self.prev_len, n_steps,
@@ -243,41 +284,82 @@ class SearchNode:
         """Enqueue dedispersion + detection of the block just pushed (centres
         [T0 - 64, T0 + L - 64)) and the copy of its candidate records to host
         memory; :meth:`finish` waits for them and decodes."""
-        cfg = self.cfg
         s = self._stream(stream)
         T0 = self.T0
         c_lo = T0 - DETECT_DELAY_STEPS
         plan, pd = self._plan_for(n_steps)
         counts = self._block_counts(c_lo, n_steps)
         cand, n_cand = self.cand_slots[slot], self.n_cand_slots[slot]
+        si, so = self.stat_cur, (self.stat_cur + 1) % self.n_stat
+        seq = self.block_seq
         with torch.cuda.stream(s):
             self._blk_pinned[slot].copy_(torch.from_numpy(counts))
             self.blk_count.copy_(self._blk_pinned[slot], non_blocking=True)
             n_cand.zero_()
             self._mark("dedisp", 0)
-            self._launch_dedisp(plan, pd, self.mean, self.m2, cand, n_cand, T0, n_steps, s)
+            self._launch_dedisp(plan, pd, si, self.mean_bufs[so], self.m2_bufs[so], cand, n_cand, T0,
+                                n_steps, s, float(self.cfg.threshold), self.cur)
             self._mark("dedisp", 1)
-            self.n_prev += self.blk_count
+            torch.add(self.n_prev_bufs[si], self.blk_count, out=self.n_prev_bufs[so])
             self._cnt_pinned[slot].copy_(n_cand, non_blocking=True)
             self._rec_pinned[slot].copy_(cand[:self._rec_prefix], non_blocking=True)
             done = torch.cuda.Event()
             done.record(s)
+        self.stat_writer[so] = seq
+        self.stat_cur = so
         self.n_prev_host += counts
         self.T0 = T0 + n_steps
-        return PendingBlock(slot=slot, T0=T0, n_steps=n_steps, done=done, stream=s)
+        self.block_seq = seq + 1
+        return PendingBlock(slot=slot, T0=T0, n_steps=n_steps, done=done, stream=s, seq=seq, stat_in=si,
+                            ring=self.cur, plan_key=n_steps)
 
-    def _launch_dedisp(self, plan, pd, mean, m2, cand, n_cand, T0, n_steps, s) -> None:
+    def _launch_dedisp(self, plan, pd, stat_in, mean_out, m2_out, cand, n_cand, T0, n_steps, s,
+                       threshold, ring, cand_cap=None) -> None:
+        """One ``dsx_dedisp_detect`` launch on history buffer ``ring`` against
+        state buffer ``stat_in``; ``mean_out`` None detects without merging,
+        ``cand_cap`` 0 only counts records."""
         cfg = self.cfg
-        _lib.call("dsx_dedisp_detect", _lib.ptr(self.rings[self.cur]),
+        _lib.call("dsx_dedisp_detect", _lib.ptr(self.rings[ring]),
                   _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
-                  cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(pd["sub_info"]),
-                  _lib.ptr(pd["rows"]) if plan.n_rows else None, plan.n_rows, _lib.ptr(pd["tbase"]),
-                  plan.n_terms, cfg.n_dm, _lib.ptr(self.valid_from), _lib.ptr(self.n_prev),
-                  _lib.ptr(self.blk_count), _lib.ptr(mean), _lib.ptr(m2), T0 - DETECT_DELAY_STEPS, n_steps,
-                  self.n_widths, plan.tile, plan.r_total, plan.q_total, float(cfg.threshold),
-                  float(cfg.min_stat_count), _lib.ptr(cand), self.cand_cap, _lib.ptr(n_cand), _lib.stream(s))
+                  cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(self.shift),
+                  _lib.ptr(pd["sub_info"]), _lib.ptr(pd["rows"]) if plan.n_rows else None, plan.n_rows,
+                  _lib.ptr(pd["tbase"]), plan.n_terms, cfg.n_dm, _lib.ptr(self.valid_from),
+                  _lib.ptr(self.n_prev_bufs[stat_in]), _lib.ptr(self.blk_count),
+                  _lib.ptr(self.mean_bufs[stat_in]), _lib.ptr(self.m2_bufs[stat_in]),
+                  _lib.ptr(mean_out), _lib.ptr(m2_out), T0 - DETECT_DELAY_STEPS, n_steps,
+                  self.n_widths, plan.tile, plan.r_total, plan.q_total, float(threshold),
+                  float(cfg.min_stat_count), _lib.ptr(cand), self.cand_cap if cand_cap is None else int(cand_cap),
+                  _lib.ptr(n_cand), _lib.stream(s))
 
-    def autotune(self, n_steps: int, *, budgets=(16 << 10, 24 << 10, 32 << 10, 48 << 10, 64 << 10, 80 << 10, 112 << 10, 160 << 10),
+    AUTOTUNE_BUDGETS = (16 << 10, 24 << 10, 32 << 10, 48 << 10, 64 << 10, 80 << 10, 112 << 10, 160 << 10)
+
+    def layouts(self, n_steps: int, modes: Optional[tuple] = None, budgets=None) -> list:
+        """Every distinct dedispersion LDS layout (time tile, pair rows, two
+        window buffers) the planner offers for blocks of ``n_steps`` over the
+        autotuner's LDS budgets."""
+        if modes is None:
+            modes = (2, 1, 3) if self.ring_dtype == torch.float32 else (2, 1)
+        kw = dict(n_widths=self.n_widths, n_centres=n_steps, ring_dtype=self.ring_dtype)
+        seen, out = set(), []
+        for mode in modes:
+            for b in (budgets or self.AUTOTUNE_BUDGETS):
+                try:
+                    plan = plan_dedisp(self.shifts_host.T, lds_budget=b, mode=mode, **kw)
+                except RuntimeError:
+                    continue
+                key = (plan.tile, plan.paired, plan.q_total > 0 and not plan.paired)
+                if key not in seen:
+                    seen.add(key)
+                    out.append(plan)
+        return out
+
+    def use_plan(self, n_steps: int, plan) -> None:
+        """Detect blocks of ``n_steps`` with ``plan`` (a :class:`DedispPlan`)."""
+        pd = {k: torch.from_numpy(np.ascontiguousarray(getattr(plan, k))).to(self.dev)
+              for k in ("sub_info", "rows", "tbase")}
+        self.plans[n_steps] = (plan, pd)
+
+    def autotune(self, n_steps: int, *, budgets=AUTOTUNE_BUDGETS,
                  modes: Optional[tuple] = None, reps: int = 2) -> List[dict]:
         """Time the dedispersion kernel under each LDS layout the planner
         offers (time tile from the LDS budget; unpaired or subband-pair rows)
@@ -287,7 +369,7 @@ class SearchNode:
         if modes is None:
             modes = (2, 1, 3) if self.ring_dtype == torch.float32 else (2, 1)
         s = self._stream(None)
-        mean, m2 = self.mean.clone(), self.m2.clone()
+        mean, m2 = torch.empty_like(self.mean), torch.empty_like(self.m2)
         cand = torch.empty_like(self.cand_slots[0])
         n_cand = torch.zeros(1, dtype=torch.int32, device=self.dev)
         seen, rows, best = set(), [], None
@@ -308,11 +390,10 @@ class SearchNode:
                 with torch.cuda.stream(s):
                     for _ in range(reps):
                         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                        mean.copy_(self.mean)
-                        m2.copy_(self.m2)
                         n_cand.zero_()
                         e0.record(s)
-                        self._launch_dedisp(plan, pd, mean, m2, cand, n_cand, self.T0, n_steps, s)
+                        self._launch_dedisp(plan, pd, self.stat_cur, mean, m2, cand, n_cand, self.T0, n_steps, s,
+                                            float(self.cfg.threshold), self.cur)
                         e1.record(s)
                         e1.synchronize()
                         ms.append(e0.elapsed_time(e1))
@@ -327,21 +408,79 @@ class SearchNode:
         return rows
 
     def finish(self, pend: "PendingBlock") -> BlockResult:
-        """Wait for a block's detection and decode its candidates."""
+        """Wait for a block's detection and decode its candidates.
+
+        Overflow policy: when more chunk maxima beat the threshold than the
+        record buffer holds, the kept records would depend on the order of
+        the kernel's atomic appends.  The block is then detected again
+        (records only, no statistics merge) against the same history and
+        state at the lowest threshold of the ladder ``threshold *
+        sqrt(2)**j`` whose records all fit, so the result is deterministic and
+        keeps every strongest record; ``threshold_used`` reports that
+        threshold and ``n_raw`` the uncapped count at the node's threshold."""
         cfg = self.cfg
         pend.done.synchronize()
         n = int(self._cnt_pinned[pend.slot][0])
+        thr = float(cfg.threshold)
+        c_lo = pend.T0 - DETECT_DELAY_STEPS
         k = min(n, self.cand_cap)
         if k <= self._rec_prefix:
-            rec = self._rec_pinned[pend.slot][:k].numpy().copy().view(CAND_DTYPE).reshape(-1)
+            rec_dev = self._rec_pinned[pend.slot][:k].numpy().copy().view(CAND_DTYPE).reshape(-1)
         else:
             with torch.cuda.stream(pend.stream):
-                rec = self.cand_slots[pend.slot][:k].cpu().numpy().view(CAND_DTYPE).reshape(-1)
+                rec_dev = self.cand_slots[pend.slot][:k].cpu().numpy().view(CAND_DTYPE).reshape(-1)
+        if n > self.cand_cap and thr > 0 and self._can_redetect(pend):
+            thr, rec_dev = self._redetect(pend, n, rec_dev)
+        rec = device_records(rec_dev, c_lo)
         cands = decode_chunk_maxima(rec, widths=cfg.widths, n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
-                                    threshold=cfg.threshold, n_top=cfg.n_top, radius_t=cfg.radius_t,
+                                    threshold=thr, n_top=cfg.n_top, radius_t=cfg.radius_t,
                                     radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm, t_int=cfg.t_int)
         return BlockResult(candidates=cands, n_raw=n, overflow=n > self.cand_cap, T0=pend.T0,
-                           n_steps=pend.n_steps, records=rec)
+                           n_steps=pend.n_steps, records=rec, threshold_used=thr)
+
+    def _redetect(self, pend: "PendingBlock", n: int, kept: np.ndarray):
+        """Lowest ladder threshold (j >= 1) whose records fit, and its records.
+
+        The kept records are a sample of the block's records, so the fraction
+        above each threshold estimates where to probe first; count-only passes
+        then walk the ladder until level j fits and level j - 1 does not (or
+        j = 1), which makes the result independent of the sample, and a last
+        pass keeps the records of level j."""
+        levels = record_levels(self.cfg.threshold)
+        frac = np.array([(kept["snr"] > lv).mean() if kept.size else 1.0 for lv in levels])
+        est = n * frac
+        fits = [j for j in range(1, N_LEVELS) if est[j] <= 0.9 * self.cand_cap]
+        j = fits[0] if fits else N_LEVELS - 1
+        plan, pd = self._plan_for(pend.plan_key)
+        cand, n_cand = self.cand_slots[pend.slot], self.n_cand_slots[pend.slot]
+
+        def probe(jj, keep):
+            with torch.cuda.stream(pend.stream):
+                n_cand.zero_()
+                self._launch_dedisp(plan, pd, pend.stat_in, None, None, cand, n_cand, pend.T0, pend.n_steps,
+                                    pend.stream, float(levels[jj]), pend.ring, None if keep else 0)
+                return int(n_cand[0].item())
+
+        counts = {}
+
+        def count(jj):
+            if jj not in counts:
+                counts[jj] = probe(jj, False)
+            return counts[jj]
+
+        while j < N_LEVELS - 1 and count(j) > self.cand_cap:
+            j += 1
+        while j > 1 and count(j - 1) <= self.cand_cap:
+            j -= 1
+        counts[j] = probe(j, True)
+        m = min(counts[j], self.cand_cap)
+        with torch.cuda.stream(pend.stream):
+            rec_dev = cand[:m].cpu().numpy().view(CAND_DTYPE).reshape(-1)
+        return float(levels[j]), rec_dev
+
+    def _can_redetect(self, pend: "PendingBlock") -> bool:
+        """The history and state buffers block ``pend`` read are unchanged."""
+        return self.ring_writer[pend.ring] == pend.seq and self.stat_writer[pend.stat_in] < pend.seq
 
     def detect(self, n_steps: int) -> BlockResult:
         """Dedisperse + detect the block just pushed and wait for its candidates."""
@@ -390,6 +529,10 @@ class PendingBlock:
     done: torch.cuda.Event
     stream: torch.cuda.Stream
     img_free: Optional[torch.cuda.Event] = None
+    seq: int = 0                 # block number on its node
+    stat_in: int = 0             # state buffer the block was detected against
+    ring: int = 0                # history buffer it read
+    plan_key: int = 0
 
 
 class MultiRangeNode:
@@ -459,20 +602,22 @@ class MultiRangeNode:
         recs, T0 = [], None
         k_off = n_raw = 0
         overflow = False
+        thr_used = []
         for (nd, f, off, _), pend in zip(self.ranges, pends):
             res = nd.finish(pend)
+            thr_used.append(res.threshold_used)
             n_raw += res.n_raw
             overflow = overflow or res.overflow
             T0 = pend.T0 * f if T0 is None else T0
             r = res.records.copy()
-            r["t"] = r["t"] * f
+            r["t"] = r["t"] * f                 # int64
             r["d"] = r["d"] + off
             r["k"] = r["k"] + k_off
             recs.append(r)
             k_off += len(cfg.widths)
-        rec = np.concatenate(recs) if recs else np.zeros(0, CAND_DTYPE)
+        rec = np.concatenate(recs) if recs else np.zeros(0, REC_DTYPE)
         cands = decode_chunk_maxima(rec, widths=self.widths_all, n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
                                     threshold=cfg.threshold, n_top=cfg.n_top, radius_t=cfg.radius_t,
                                     radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm, t_int=cfg.t_int)
         return BlockResult(candidates=cands, n_raw=int(n_raw), overflow=overflow, T0=int(T0), n_steps=n_steps,
-                           records=rec)
+                           records=rec, threshold_used=max(thr_used), range_thresholds=thr_used)

@@ -29,7 +29,7 @@ import torch
 
 from .config import DETECT_CHUNK_STEPS, DETECT_DELAY_STEPS, NodeConfig
 from .corr import unpack_int4_split
-from .detector import CAND_DTYPE, decode_chunk_maxima
+from .detector import REC_DTYPE, decode_chunk_maxima
 from .imager import compute_edge_mask
 from .pattern import build_cell_map, upper_tri_indices
 from .pipeline import BlockResult
@@ -173,7 +173,7 @@ class PlainNode:
             hit = scored & (best > cfg.threshold)
             if bool(hit.any()):
                 ci, pi = torch.nonzero(hit, as_tuple=True)
-                r = np.zeros(int(ci.numel()), dtype=CAND_DTYPE)
+                r = np.zeros(int(ci.numel()), dtype=REC_DTYPE)
                 r["t"] = (c_lo + ci * chunk + best_t[ci, pi]).cpu().numpy()
                 r["p"] = pi.cpu().numpy()
                 r["d"] = d
@@ -193,7 +193,7 @@ class PlainNode:
         self.mean = torch.where(upd, self.mean + delta * (nb / safe_n), self.mean)
         self.m2 = torch.where(upd, self.m2 + m2_b + delta * delta * (n_a * nb / safe_n), self.m2)
         self.n_prev += counts
-        rec = np.concatenate(recs) if recs else np.zeros(0, CAND_DTYPE)
+        rec = np.concatenate(recs) if recs else np.zeros(0, REC_DTYPE)
         cands = decode_chunk_maxima(rec, widths=widths, n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
                                     threshold=cfg.threshold, n_top=cfg.n_top, radius_t=cfg.radius_t,
                                     radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm, t_int=cfg.t_int)

@@ -85,6 +85,7 @@ class NodeServer:
         self.slot_free: Dict[tuple, Optional[torch.cuda.Event]] = {}
         self.owner: Dict[tuple, Optional[_Inflight]] = {}
         self.bytes_inflight = 0
+        self.budget_waits = 0                      # admissions that waited for HBM, not for a slot
         self.done: List[BlockDone] = []
         self.t0 = time.perf_counter()
 
@@ -99,11 +100,7 @@ class NodeServer:
 
     def resident_bytes(self) -> int:
         """Device bytes held by the node streams (histories, slots, state)."""
-        tot = 0
-        for nd in self.nodes:
-            for t in [*nd.rings, *nd.vis_slots, *nd.img_slots, *nd.cand_slots, nd.mean, nd.m2]:
-                tot += t.numel() * t.element_size()
-        return tot
+        return sum(nd.device_bytes() for nd in self.nodes)
 
     # ------------------------------------------------------------------ stages
     def _retire(self, block: bool) -> bool:
@@ -132,9 +129,13 @@ class NodeServer:
         slot = self.next_slot[req.stream_id]
         key = (req.stream_id, slot)
         # wait until the slot is free and the block fits the budget
+        waited_for_memory = False
         while self.owner.get(key) is not None or \
                 self.resident_bytes() + self.bytes_inflight + nbytes > self.budget:
+            if self.owner.get(key) is None:
+                waited_for_memory = True
             self._retire(block=True)
+        self.budget_waits += int(waited_for_memory)
         t_admit = self.now()
         n_steps = nd._check_block(req.n_samp)
         free = self.slot_free.get(key)

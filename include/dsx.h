@@ -166,15 +166,25 @@ int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, int n_widths
  * (first centre - w_max/2 >= valid_from[d]) and each DM with
  * n_prev[d] >= min_count, the chunk maximum over widths (ties -> lowest k,
  * then earliest centre) is appended to cand (16-byte records
- * {int32 t, int32 p, int16 d, int16 k, float snr}) when snr > threshold,
- * counted by *n_cand (device int32; entries past cand_cap are dropped but
- * still counted).  Samples t in [c_lo, c_lo + n_centres) with
- * t >= valid_from[d] feed the block's sum / sum of squares, merged (Chan)
- * into the float64 running mean / m2 ([p][d] layout) with blk_count[d]
- * samples.  sub_info, rows, tbase, n_rows, n_terms, tile, r_total and
- * q_total come from dsx_dedisp_plan (copied to device memory; rows may be
- * NULL when n_rows == 0; an unpaired plan with q_total == r_total selects the
- * two-buffer layout of mode 3 and needs ring_dtype DSX_F32).  ring_dtype DSX_F16 or DSX_F32;
+ * {int32 t - c_lo, int32 p, int16 d, int16 k, float snr}: the record's time
+ * is relative to c_lo, so it fits int32 however long the stream runs) when
+ * snr > threshold, counted by *n_cand (device int32 the caller zeroes;
+ * entries past cand_cap are dropped but still counted, and cand_cap 0 only
+ * counts).  Detection with mean_out = NULL leaves the state unchanged, so a
+ * caller whose records overflowed can detect the block again at a higher
+ * threshold.
+ * Samples t in [c_lo, c_lo + n_centres) with t >= valid_from[d] feed the
+ * block's sums, merged (Chan) with the float64 running mean / m2 ([p][d]
+ * layout, n_prev[d] samples) into mean_out / m2_out (which may alias mean /
+ * m2) with blk_count[d] samples; mean_out = m2_out = NULL detects without
+ * merging.  Samples are accumulated in float32 centred on the running mean
+ * (or, before a DM has statistics, on one of its complete samples), so the
+ * sums keep their precision for pixels that hold a steady source.
+ * shift is the device [s][d] int32 table of the plan.  sub_info, rows, tbase,
+ * n_rows, n_terms, tile, r_total and q_total come from dsx_dedisp_plan
+ * (copied to device memory; rows may be NULL when n_rows == 0; an unpaired
+ * plan with q_total == r_total selects the two-buffer layout of mode 3 and
+ * needs ring_dtype DSX_F32).  ring_dtype DSX_F16 or DSX_F32;
  * hist >= 64 + w_max/2 + max shift.
  * Replaces combine_chgroups + boxcar_via_cumsum + multi_boxcar_argmax +
  * the Layer-1/Layer-2 normalisation, src/dsart/fine_dm/combiner.py:548-655,
@@ -182,12 +192,13 @@ int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, int n_widths
  * src/dsart/detector/triton_boxcar.py:193-403,
  * src/dsart/noise_norm/layer2.py:313-452. */
 int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, int n_pix, int hist,
-                      int ring_len, int64_t T0, const int32_t *sub_info, const int32_t *rows,
-                      int n_rows, const int32_t *tbase, int n_terms, int n_dm,
+                      int ring_len, int64_t T0, const int32_t *shift, const int32_t *sub_info,
+                      const int32_t *rows, int n_rows, const int32_t *tbase, int n_terms, int n_dm,
                       const int32_t *valid_from, const double *n_prev, const double *blk_count,
-                      double *mean, double *m2, int64_t c_lo, int n_centres, int n_widths,
-                      int tile, int r_total, int q_total, float threshold, double min_count,
-                      void *cand, int cand_cap, int32_t *n_cand, dsx_stream_t stream);
+                      const double *mean, const double *m2, double *mean_out, double *m2_out,
+                      int64_t c_lo, int n_centres, int n_widths, int tile, int r_total, int q_total,
+                      float threshold, double min_count, void *cand, int cand_cap, int32_t *n_cand,
+                      dsx_stream_t stream);
 
 /* Shift-and-add combine of n_streams float32 streams (n_streams, t_stream,
  * n_inner): out[t][k] = sum_g streams[g][t0 + t - shift[g]][k] for

@@ -72,3 +72,36 @@ def correlate(raw: np.ndarray, *, n_pkt: int, n_ant: int, n_chan: int, t_int: in
     out_re = vre[..., a_idx, b_idx]
     out_im = vim[..., a_idx, b_idx]
     return np.ascontiguousarray(np.stack([out_re, out_im], axis=-1), dtype=np.int64)
+
+
+def correlate_blas(raw: np.ndarray, *, n_pkt: int, n_ant: int, n_chan: int, t_int: int,
+                   csf: int = 1) -> np.ndarray:
+    """Same visibilities as :func:`correlate` (Stokes I) through float32
+    matrix products: samples are integers in [-8, 7], so every product and
+    every partial sum (at most 2 * 64 * K for K = 2 * t_int * csf terms) is an
+    integer below 2**24 while K <= 131072, and the float32 products are exact.
+    Equality with the einsum form is checked in ``tests/test_oracle_corr.py``;
+    this twin exists only to make 96-antenna oracle streams fast enough for
+    the GPU parity tests."""
+    K = t_int * csf * 2
+    if 2 * 64 * K >= (1 << 24):
+        raise ValueError("float32 products would not be exact for this K")
+    b = np.arange(256)
+    lut_re = (((b & 0xF) ^ 8) - 8).astype(np.float32)
+    lut_im = ((((b >> 4) & 0xF) ^ 8) - 8).astype(np.float32)
+    raw = np.asarray(raw, dtype=np.uint8)
+    n_steps = n_pkt * 2 // t_int
+    nco = n_chan // csf
+    # on-wire (pkt, ant, chan, t2, pol) -> (step, cout, [pkt in step, t2, x, pol], ant)
+    shp = (n_steps, t_int // 2, n_ant, nco, csf, 2, 2)
+    order = (0, 3, 1, 5, 4, 6, 2)
+    R = np.ascontiguousarray(lut_re[raw].reshape(shp).transpose(order)).reshape(n_steps, nco, K, n_ant)
+    I = np.ascontiguousarray(lut_im[raw].reshape(shp).transpose(order)).reshape(n_steps, nco, K, n_ant)
+    Rt = np.swapaxes(R, -1, -2)
+    It = np.swapaxes(I, -1, -2)
+    # [a, b] = sum_k X[k, a] Y[k, b]
+    vre = Rt @ R + It @ I
+    vim = It @ R - Rt @ I
+    a_idx, b_idx = upper_tri(n_ant)
+    out = np.stack([vre[..., a_idx, b_idx], vim[..., a_idx, b_idx]], axis=-1)
+    return out.astype(np.int64)

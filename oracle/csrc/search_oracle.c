@@ -33,7 +33,7 @@ int oracle_search_block(
     int64_t c_lo, int n_centres, int chunk,
     const int32_t *widths, int n_w,
     const double *st_n, const double *st_mean, const double *st_m2, double min_count,
-    double *best_snr, int32_t *best_t, int32_t *best_k,
+    double *best_snr, int64_t *best_t, int32_t *best_k,
     double *blk_sum, double *blk_sumsq, double *blk_count)
 {
     if (n_centres % chunk) return -1;
@@ -118,7 +118,7 @@ int oracle_search_block(
                 }
                 const size_t o = ((size_t)d * n_chunks + ch) * n_pix + p;
                 best_snr[o] = bsnr;
-                best_t[o] = bt < 0 ? -1 : (int32_t)(c_lo + bt);
+                best_t[o] = bt < 0 ? -1 : c_lo + bt;
                 best_k[o] = bk;
             }
         }

@@ -103,7 +103,7 @@ def search_block_np(img, tau0, shifts, c_lo, n_centres, chunk, widths, st_n, st_
     n_dm = shifts.shape[0]
     n_chunks = n_centres // chunk
     best_snr = np.full((n_dm, n_chunks, n_pix), -np.inf)
-    best_t = np.full((n_dm, n_chunks, n_pix), -1, dtype=np.int32)
+    best_t = np.full((n_dm, n_chunks, n_pix), -1, dtype=np.int64)
     best_k = np.full((n_dm, n_chunks, n_pix), -1, dtype=np.int32)
     cent = D[:, half:half + n_centres]
     valid_from = shifts.max(axis=1).astype(np.int64)
@@ -140,8 +140,16 @@ def search_block_np(img, tau0, shifts, c_lo, n_centres, chunk, widths, st_n, st_
 def search_block_c(img, tau0, shifts, c_lo, n_centres, chunk, widths, st_n, st_mean, st_m2,
                    min_count=512):
     """C inner loop; ``img`` is ``(n_sub, n_tau, n_pix)``, transposed to time-contiguous."""
-    n_sub, n_tau, n_pix = img.shape
     im = np.ascontiguousarray(np.transpose(img, (0, 2, 1)), dtype=np.float64)
+    return search_block_c_tmajor(im, tau0, shifts, c_lo, n_centres, chunk, widths, st_n, st_mean, st_m2,
+                                 min_count)
+
+
+def search_block_c_tmajor(im, tau0, shifts, c_lo, n_centres, chunk, widths, st_n, st_mean, st_m2,
+                          min_count=512):
+    """:func:`search_block_c` on a time-contiguous history ``im`` ``(n_sub, n_pix, n_tau)``."""
+    n_sub, n_pix, n_tau = im.shape
+    im = np.ascontiguousarray(im, dtype=np.float64)
     n_dm = shifts.shape[0]
     n_chunks = n_centres // chunk
     sh = np.ascontiguousarray(shifts, dtype=np.int32)
@@ -150,7 +158,7 @@ def search_block_c(img, tau0, shifts, c_lo, n_centres, chunk, widths, st_n, st_m
     stm = np.ascontiguousarray(st_mean, dtype=np.float64)
     st2 = np.ascontiguousarray(st_m2, dtype=np.float64)
     bs = np.empty((n_dm, n_chunks, n_pix))
-    bt = np.empty((n_dm, n_chunks, n_pix), dtype=np.int32)
+    bt = np.empty((n_dm, n_chunks, n_pix), dtype=np.int64)
     bk = np.empty((n_dm, n_chunks, n_pix), dtype=np.int32)
     s1 = np.empty((n_dm, n_pix))
     s2 = np.empty((n_dm, n_pix))

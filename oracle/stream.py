@@ -53,7 +53,7 @@ class OracleStream:
         self.st_n = np.zeros(cfg.n_dm)
         self.st_mean = np.zeros((cfg.n_dm, npx))
         self.st_m2 = np.zeros((cfg.n_dm, npx))
-        self.hist = np.zeros((cfg.n_sub, 0, npx))
+        self.hist = np.zeros((cfg.n_sub, npx, 0))        # time-contiguous per pixel
         self.hist_tau0 = 0
         self.t_next = 0                  # first step of the next block
         self.keep = int(self.shifts.max()) + self.half + DELAY + 1
@@ -65,24 +65,34 @@ class OracleStream:
                                t_int=c.t_int, csf=c.csf, fuse_pol=True)
 
     def images(self, vis: np.ndarray) -> np.ndarray:
-        """``(n_sub, n_steps, n_pix_sel)`` float64."""
+        """``(n_sub, n_steps, n_pix_sel)`` float64 (the sparse-product twin of
+        ``oracle.image.subband_images``, equal to it on integer visibilities)."""
         c = self.cfg
-        img = oimage.subband_images(vis, self.rows, self.cols, c.n_grid, c.n_sub, self.mask)
-        return img.reshape(c.n_sub, img.shape[1], -1)[:, :, self.pix]
+        return oimage.subband_images_sparse(vis, self.rows, self.cols, c.n_grid, c.n_sub, self.mask,
+                                            pix=self.pix)
+
+    def start_at(self, t0: int) -> None:
+        """Start the stream at step ``t0`` with zeros before it (as a node
+        whose history buffers start zeroed)."""
+        self.t_next = int(t0)
+        self.hist = np.zeros((self.cfg.n_sub, self.pix.size, self.keep))
+        self.hist_tau0 = int(t0) - self.keep
 
     def push_images(self, img: np.ndarray):
-        """Append a block of images; run detection; merge statistics."""
+        """Append a block of images ``(n_sub, L, n_pix_sel)``; run detection;
+        merge statistics."""
         c = self.cfg
         L = img.shape[1]
         T0 = self.t_next
-        self.hist = np.concatenate([self.hist, img], axis=1)
-        drop = max(0, self.hist.shape[1] - (self.keep + L))
-        if drop:
-            self.hist = self.hist[:, drop:]
-            self.hist_tau0 += drop
+        keep_old = min(self.hist.shape[2], self.keep)
+        new = np.empty((c.n_sub, self.pix.size, keep_old + L))
+        new[:, :, :keep_old] = self.hist[:, :, self.hist.shape[2] - keep_old:]
+        new[:, :, keep_old:] = np.transpose(img, (0, 2, 1))
+        self.hist_tau0 += self.hist.shape[2] - keep_old
+        self.hist = new
         self.t_next = T0 + L
         c_lo = T0 - DELAY
-        bs, bt, bk, s1, s2, cnt = osearch.search_block_c(
+        bs, bt, bk, s1, s2, cnt = osearch.search_block_c_tmajor(
             self.hist, self.hist_tau0, self.shifts, c_lo, L, CHUNK, self.widths,
             self.st_n, self.st_mean, self.st_m2, getattr(c, "min_stat_count", 512))
         self.last = dict(best_snr=bs, best_t=bt, best_k=bk, blk_sum=s1, blk_sumsq=s2, count=cnt,
@@ -96,5 +106,11 @@ class OracleStream:
         return cands
 
     def push_raw(self, raw: np.ndarray, n_samp: int):
-        vis = self.visibilities(raw, n_samp)
+        vis = self.visibilities_fast(raw, n_samp)
         return self.push_images(self.images(vis))
+
+    def visibilities_fast(self, raw: np.ndarray, n_samp: int) -> np.ndarray:
+        """:meth:`visibilities` through the float32-product twin (equal integers)."""
+        c = self.cfg
+        return ocorr.correlate_blas(raw, n_pkt=n_samp // 2, n_ant=c.n_ant, n_chan=len(c.chan_freqs_GHz),
+                                    t_int=c.t_int, csf=c.csf)

@@ -110,3 +110,19 @@ def test_pol_split_sums_to_stokes_i():
     per = ocorr.correlate(raw, n_pkt=S // 2, n_ant=A, n_chan=C, t_int=8, fuse_pol=False)
     fused = ocorr.correlate(raw, n_pkt=S // 2, n_ant=A, n_chan=C, t_int=8)
     assert np.array_equal(per.sum(axis=2), fused)
+
+
+def test_blas_twin_equals_einsum_form():
+    """The float64-product twin used for 96-antenna oracle streams returns
+    the einsum form's integers exactly (channel sums and both pols fused)."""
+    rng = np.random.default_rng(5)
+    n_pkt, n_ant, n_chan = 16, 7, 12
+    raw = rng.integers(0, 256, size=(n_pkt, n_ant, n_chan, 2, 2), dtype=np.uint8)
+    for t_int, csf in ((2, 1), (8, 3), (4, 12)):
+        want = ocorr.correlate(raw, n_pkt=n_pkt, n_ant=n_ant, n_chan=n_chan, t_int=t_int, csf=csf)
+        got = ocorr.correlate_blas(raw, n_pkt=n_pkt, n_ant=n_ant, n_chan=n_chan, t_int=t_int, csf=csf)
+        assert got.dtype == np.int64 and np.array_equal(got, want), (t_int, csf)
+    # extreme codes (-8 everywhere) reach the largest partial sums
+    raw8 = np.full((n_pkt, n_ant, n_chan, 2, 2), 0x88, dtype=np.uint8)
+    assert np.array_equal(ocorr.correlate_blas(raw8, n_pkt=n_pkt, n_ant=n_ant, n_chan=n_chan, t_int=8, csf=4),
+                          ocorr.correlate(raw8, n_pkt=n_pkt, n_ant=n_ant, n_chan=n_chan, t_int=8, csf=4))

@@ -103,3 +103,25 @@ def test_grid_plane_is_linear():
     g = lambda v: oimage.grid_plane(v, rows, cols, 8)
     np.testing.assert_allclose(g(2 * v1 - 3j * v2), 2 * g(v1) - 3j * g(v2), atol=1e-12)
     assert np.isclose(g(v1).sum(), v1[rows >= 0].sum())
+
+
+def test_sparse_twin_equals_add_at_form():
+    """The sparse-product / scipy.fft twin used for 96-antenna oracle streams
+    equals the np.add.at + numpy ifft2 images (integer visibilities, several
+    baselines per cell, dropped cells, a pixel subset)."""
+    rng = np.random.default_rng(9)
+    n_ant, n_chan, n_sub, N, steps = 9, 8, 2, 16, 3
+    e = rng.uniform(-300, 300, n_ant)
+    n = rng.uniform(-80, 80, n_ant)
+    freqs = np.linspace(1.49, 1.40, n_chan)
+    rows, cols, _ = oimage.cell_map(e, n, freqs, N, cell_lambda=None)
+    rows[3, 2] = cols[3, 2] = -1
+    nb = rows.shape[0]
+    vis = rng.integers(-50, 50, size=(steps, n_chan, nb, 2)).astype(np.int64)
+    mask = oimage.edge_mask(N, 1)
+    want = oimage.subband_images(vis, rows, cols, N, n_sub, mask).reshape(n_sub, steps, N * N)
+    got = oimage.subband_images_sparse(vis, rows, cols, N, n_sub, mask)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    pix = np.array([0, 17, 100, 255])
+    np.testing.assert_allclose(oimage.subband_images_sparse(vis, rows, cols, N, n_sub, mask, pix=pix),
+                               want[:, :, pix], rtol=0, atol=1e-12 * np.abs(want).max())
