@@ -16,13 +16,11 @@
 //
 // Centring.  Each lane subtracts a bias b from every dedispersed sample,
 // D' = D - b, where b is its float64 running mean rounded to 11 significant
-// bits (or, in a lane's first block, the mean of up to 64 complete samples of
-// this block read from the history, rounded the same way).  Prefix sums, boxcars and the block's statistics are
-// accumulated on D', so their float32 magnitudes stay at the noise level even
-// for pixels holding a steady source; the float64 merge and the S/N add b
-// back exactly.  Lanes not yet scored sum their block statistics directly
-// rather than from prefix differences, since a rough first-block bias lets
-// their prefix drift.
+// bits (before a DM's first block, dsx_dedisp_prime sets that mean to the
+// block's own mean).  Prefix sums, boxcars
+// and the block's statistics are accumulated on D', so their float32
+// magnitudes stay at the noise level even for pixels holding a steady
+// source; the float64 merge and the S/N add b back exactly.
 //
 // Each lane sweeps time in 64-centre chunks and keeps the exclusive prefix
 // sum P of its centred series in a register window
@@ -56,7 +54,6 @@ static_assert(sizeof(Cand) == 16, "candidate record must be 16 bytes");
 
 struct DedispParams {
     const void *ring;
-    const int32_t *shift;       // [s][d] steps (a lane's first-block bias sample)
     const int32_t *sub_info;    // [s][3] = (R offset, shmax_s, window length L_s)
     const int32_t *rows;        // [r][6] = (Q offset, length, R index of a at e = 0,
                                 //           R index of b at e = 0, b's R bounds [lo, hi); empty
@@ -65,16 +62,15 @@ struct DedispParams {
     const int32_t *valid_from;  // [d]
     const double *n_prev;       // [d]
     const double *blk_count;    // [d]
-    const double *mean;         // [p][d] state of earlier blocks
-    const double *m2;           // [p][d]
-    double *mean_out;           // [p][d] merged state (may alias mean; NULL: no merge)
-    double *m2_out;
+    double *mean;               // [p][d] state of earlier blocks, merged in place when `merge`
+    double *m2;                 // [p][d]
     Cand *cand;
     int32_t *n_cand;            // records appended (kept or not)
     int64_t T0, c_lo;
     int n_sub, n_pix, hist, row_len, n_dm, n_rows, n_terms, n_centres, tt, r_total, q_total,
         cand_cap;
     float threshold;
+    int merge;                  // 0: detect only, the state is left unchanged
     double min_count;
 };
 
@@ -95,7 +91,6 @@ constexpr int DD_THREADS = 256;   // DMs per work-group (grid.y covers the rest)
 constexpr int DD_MAX_ROWS = 4096;
 constexpr int DD_STAGE_ITERS = 12;   // fast path: a subband window spans at most 12 x 64 steps
 constexpr int DD_STAGE_SUBS = 4;     // fast path: subbands a wave stages per batch of loads
-constexpr int DD_BIAS_SAMPLES = 64;  // samples behind a lane's first-block centring bias
 
 // Centred D'(t) = D(t) - bias of DM d from the term rows, in the main path's
 // order; t_rel = t - (first centre of the tile).
@@ -144,11 +139,8 @@ __device__ __forceinline__ void sum_terms(float (&Pw)[NP], const E *X, const uin
     }
 }
 
-// Two-buffer instances with term bases in LDS and widths up to 32 fit 168
-// VGPRs without spills: ask for three waves per SIMD, which is what hides the
-// LDS latency of the term loop.
 template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false>
-__global__ __launch_bounds__(DD_THREADS, ASYNC && MAXT > 0 && NW <= 6 ? 3 : 2) void dedisp_kernel(DedispParams P) {
+__global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
     constexpr int WMAX = 1 << (NW - 1);
     constexpr int H2 = WMAX / 2;
     constexpr int WW = 2 * H2;
@@ -240,24 +232,7 @@ __global__ __launch_bounds__(DD_THREADS, ASYNC && MAXT > 0 && NW <= 6 ? 3 : 2) v
             const double m2_a = P.m2[(int64_t)p * P.n_dm + d];
             const double sig = n_a > 0.0 ? sqrt(m2_a / n_a) : 0.0;
             score = (n_a >= P.min_count) && (n_a > 0.0) && (sig > 0.0);
-            if (n_a > 0.0) {
-                bias = round_sig11((float)mean_a);
-            } else {
-                // no state yet: centre on the mean of this block's first (up to)
-                // DD_BIAS_SAMPLES complete samples, summed straight from the
-                // history (in bounds: c_lo <= t < c_hi and hist >= 64 + max shift)
-                const int64_t t_b = vf > P.c_lo ? (int64_t)vf : P.c_lo;
-                const int nk = (int)min((int64_t)DD_BIAS_SAMPLES, c_hi - t_b);
-                if (nk > 0) {
-                    float acc = 0.f;
-                    for (int s = 0; s < P.n_sub; ++s) {
-                        const T *row = ring + ((int64_t)s * P.n_pix + p) * P.row_len +
-                                       (P.hist + t_b - P.shift[(int64_t)s * P.n_dm + d] - P.T0);
-                        for (int k = 0; k < nk; ++k) acc += (float)row[k];
-                    }
-                    bias = round_sig11(acc / (float)nk);
-                }
-            }
+            bias = round_sig11((float)mean_a);
             rmu0 = (float)((double)bias - mean_a);
             inv0 = score ? (float)(1.0 / sig) : 0.f;
             inv1 = score ? (float)(1.0 / (sig * sqrt(2.0))) : 0.f;
@@ -409,19 +384,8 @@ __global__ __launch_bounds__(DD_THREADS, ASYNC && MAXT > 0 && NW <= 6 ? 3 : 2) v
                     float a1 = 0.f, a2 = 0.f;
                     const bool full = t1 >= vf && t1 + DD_SUB <= c_hi;
                     if (full) {
-                        if (score) {
 #pragma unroll
-                            for (int i = 0; i < DD_SUB; ++i) a2 = fmaf(Pw[WW + 1 + i], Pw[WW + 1 + i], a2);
-                        } else {
-                            // lanes not yet scored (statistics burn-in) may be centred
-                            // on a rough bias, so their prefix drifts: sum the samples
-                            // directly instead of differencing the prefix
-#pragma unroll
-                            for (int i = 0; i < DD_SUB; ++i) {
-                                a1 += Pw[WW + 1 + i];
-                                a2 = fmaf(Pw[WW + 1 + i], Pw[WW + 1 + i], a2);
-                            }
-                        }
+                        for (int i = 0; i < DD_SUB; ++i) a2 = fmaf(Pw[WW + 1 + i], Pw[WW + 1 + i], a2);
                     } else {
 #pragma unroll
                         for (int i = 0; i < DD_SUB; ++i) {
@@ -441,7 +405,7 @@ __global__ __launch_bounds__(DD_THREADS, ASYNC && MAXT > 0 && NW <= 6 ? 3 : 2) v
                     // ---- prefix sums in place
 #pragma unroll
                     for (int i = 0; i < DD_SUB; ++i) Pw[WW + 1 + i] += Pw[WW + i];
-                    if (full && score) a1 = Pw[WW + DD_SUB] - Pw[WW];
+                    if (full) a1 = Pw[WW + DD_SUB] - Pw[WW];
                     s1 += (double)a1;
                     s2 += (double)a2;
                     // ---- boxcar maxima of the wider widths over this sub-chunk's centres
@@ -523,25 +487,93 @@ __global__ __launch_bounds__(DD_THREADS, ASYNC && MAXT > 0 && NW <= 6 ? 3 : 2) v
         }
         // ---- merge this block's statistics into the running state
         // (s1, s2 are sums of the centred samples D - bias)
-        if (active && P.mean_out != nullptr) {
+        if (active && P.merge) {
             const double nb = P.blk_count[d];
-            const int64_t i = (int64_t)p * P.n_dm + d;
-            // earlier blocks' state, read again rather than kept in registers
-            const double mean_a = P.mean[i];
-            const double m2_a = P.m2[i];
             if (nb > 0.0) {
+                // earlier blocks' state, read again rather than kept in registers
+                const int64_t i = (int64_t)p * P.n_dm + d;
                 const double n_a = P.n_prev[d];
+                const double mean_a = P.mean[i];
+                const double m2_a = P.m2[i];
                 const double m2_b = s2 - s1 * s1 / nb;
                 const double n = n_a + nb;
                 const double delta = ((double)bias - mean_a) + s1 / nb;     // mean_b - mean_a
-                P.mean_out[i] = mean_a + delta * (nb / n);
-                P.m2_out[i] = m2_a + m2_b + delta * delta * (n_a * nb / n);
-            } else if (P.mean_out != P.mean) {
-                P.mean_out[i] = mean_a;
-                P.m2_out[i] = m2_a;
+                P.mean[i] = mean_a + delta * (nb / n);
+                P.m2[i] = m2_a + m2_b + delta * delta * (n_a * nb / n);
             }
         }
     }
+}
+
+// Centre of a DM's first block: for each (pixel, DM) with no state yet
+// (n_prev[d] == 0), the mean of its complete samples in the block,
+//     D(t) = sum_s row_s[hist + t - shift[s][d] - T0],  max(valid_from[d], c_lo) <= t < c_hi,
+// written as the running mean.  A work-group owns one pixel and 256 DMs and
+// walks each subband's row in chunks: the chunk's exclusive prefix sums
+// (float64, LDS) give every lane the two ends of its range, so a DM costs two
+// lookups per subband.  With no samples behind it the mean only centres the
+// block's sums; the merge weighs it by 0.
+constexpr int DD_PRIME_CHUNK = 2048;          // row positions per LDS prefix
+constexpr int DD_PRIME_PER_THREAD = DD_PRIME_CHUNK / DD_THREADS;
+
+template <typename T>
+__global__ __launch_bounds__(DD_THREADS) void dedisp_prime_kernel(const T *ring, int n_sub, int n_pix, int hist,
+                                                                  int row_len, int64_t T0, const int32_t *shift,
+                                                                  int n_dm, const int32_t *valid_from,
+                                                                  const double *n_prev, int64_t c_lo,
+                                                                  int n_centres, double *mean) {
+    __shared__ double pre[DD_PRIME_CHUNK + 1];     // pre[i] = sum of the chunk's positions < i
+    __shared__ double part[DD_THREADS];
+    const int tid = threadIdx.x;
+    const int d = blockIdx.y * DD_THREADS + tid;
+    const int p = blockIdx.x;
+    const int64_t c_hi = c_lo + n_centres;
+    bool need = d < n_dm && n_prev[d] == 0.0;
+    const int64_t t_b = need ? (valid_from[d] > c_lo ? (int64_t)valid_from[d] : c_lo) : c_hi;
+    need = need && t_b < c_hi;
+    double acc = 0.0;
+    for (int s = 0; s < n_sub; ++s) {
+        const T *row = ring + ((int64_t)s * n_pix + p) * row_len;
+        const int sh = need ? shift[(int64_t)s * n_dm + d] : 0;
+        // this lane's range of row positions [a, b)
+        const int64_t a = hist + t_b - sh - T0, b = hist + c_hi - sh - T0;
+        double carry = 0.0;
+        for (int c0 = 0; c0 < row_len; c0 += DD_PRIME_CHUNK) {
+            // exclusive prefix of positions [c0, c0 + DD_PRIME_CHUNK)
+            double v[DD_PRIME_PER_THREAD];
+            double run = 0.0;
+#pragma unroll
+            for (int k = 0; k < DD_PRIME_PER_THREAD; ++k) {
+                const int pos = c0 + tid * DD_PRIME_PER_THREAD + k;
+                v[k] = pos < row_len ? (double)row[pos] : 0.0;
+                run += v[k];
+            }
+            part[tid] = run;
+            __syncthreads();
+            for (int off = 1; off < DD_THREADS; off <<= 1) {       // inclusive scan of thread totals
+                const double x = tid >= off ? part[tid - off] : 0.0;
+                __syncthreads();
+                part[tid] += x;
+                __syncthreads();
+            }
+            double q = part[tid] - run;                              // exclusive
+#pragma unroll
+            for (int k = 0; k < DD_PRIME_PER_THREAD; ++k) {
+                pre[tid * DD_PRIME_PER_THREAD + k] = q;
+                q += v[k];
+            }
+            if (tid == DD_THREADS - 1) pre[DD_PRIME_CHUNK] = q;
+            __syncthreads();
+            if (need) {
+                if (a >= c0 && a < c0 + DD_PRIME_CHUNK) acc -= carry + pre[a - c0];
+                if (b >= c0 && b < c0 + DD_PRIME_CHUNK) acc += carry + pre[b - c0];
+            }
+            carry += pre[DD_PRIME_CHUNK];
+            __syncthreads();                                         // pre and part are reused
+        }
+        if (need && b == row_len) acc += carry;
+    }
+    if (need) mean[(int64_t)p * n_dm + d] = acc / (double)(c_hi - t_b);
 }
 
 // Plans with at most 8 (or 16) terms keep the lanes' term bases in registers.
@@ -773,19 +805,15 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
 }
 
 extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, int n_pix, int hist,
-                                 int ring_len, int64_t T0, const int32_t *shift, const int32_t *sub_info,
-                                 const int32_t *rows, int n_rows, const int32_t *tbase, int n_terms,
-                                 int n_dm, const int32_t *valid_from, const double *n_prev,
-                                 const double *blk_count, const double *mean, const double *m2,
-                                 double *mean_out, double *m2_out, int64_t c_lo, int n_centres,
+                                 int ring_len, int64_t T0, const int32_t *sub_info, const int32_t *rows,
+                                 int n_rows, const int32_t *tbase, int n_terms, int n_dm,
+                                 const int32_t *valid_from, const double *n_prev, const double *blk_count,
+                                 double *mean, double *m2, int merge, int64_t c_lo, int n_centres,
                                  int n_widths, int tile, int r_total, int q_total, float threshold,
                                  double min_count, void *cand, int cand_cap, int32_t *n_cand,
                                  dsx_stream_t stream) {
-    DSX_REQUIRE(ring && shift && sub_info && tbase && valid_from && n_prev && blk_count && mean && m2 &&
-                    cand && n_cand,
+    DSX_REQUIRE(ring && sub_info && tbase && valid_from && n_prev && blk_count && mean && m2 && cand && n_cand,
                 "dsx_dedisp_detect: null pointer");
-    DSX_REQUIRE((mean_out == nullptr) == (m2_out == nullptr),
-                "dsx_dedisp_detect: mean_out and m2_out must both be set or both be NULL");
     DSX_REQUIRE(ring_dtype == DSX_F16 || ring_dtype == DSX_F32, "dsx_dedisp_detect: ring_dtype");
     DSX_REQUIRE(n_sub > 0 && n_pix > 0 && n_dm > 0 && hist >= 0 && ring_len > hist,
                 "dsx_dedisp_detect: bad shape");
@@ -809,7 +837,6 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
         return set_error(DSX_ERANGE, "dsx_dedisp_detect: %zu B of LDS for the time windows", lds);
     DedispParams P{};
     P.ring = ring;
-    P.shift = shift;
     P.sub_info = sub_info;
     P.rows = rows;
     P.tbase = tbase;
@@ -818,8 +845,7 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
     P.blk_count = blk_count;
     P.mean = mean;
     P.m2 = m2;
-    P.mean_out = mean_out;
-    P.m2_out = m2_out;
+    P.merge = merge != 0;
     P.cand = reinterpret_cast<Cand *>(cand);
     P.n_cand = n_cand;
     P.T0 = T0;
@@ -846,4 +872,29 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
                       : dispatch_nw<_Float16, false>(n_widths, P, threads, lds, grid, s);
     return paired ? dispatch_nw<float, true>(n_widths, P, threads, lds, grid, s)
                   : dispatch_nw<float, false>(n_widths, P, threads, lds, grid, s);
+}
+
+extern "C" int dsx_dedisp_prime(const void *ring, int ring_dtype, int n_sub, int n_pix, int hist, int ring_len,
+                                int64_t T0, const int32_t *shift, int n_dm, const int32_t *valid_from,
+                                const double *n_prev, int64_t c_lo, int n_centres, double *mean,
+                                dsx_stream_t stream) {
+    DSX_REQUIRE(ring && shift && valid_from && n_prev && mean, "dsx_dedisp_prime: null pointer");
+    DSX_REQUIRE(ring_dtype == DSX_F16 || ring_dtype == DSX_F32, "dsx_dedisp_prime: ring_dtype");
+    DSX_REQUIRE(n_sub > 0 && n_pix > 0 && n_dm > 0 && hist >= 0 && ring_len > hist && n_centres > 0,
+                "dsx_dedisp_prime: bad shape");
+    // samples t in [c_lo, c_lo + n_centres) sit at ring positions hist + t - shift - T0
+    DSX_REQUIRE(hist + (c_lo - T0) >= 0 && hist + (c_lo + n_centres - T0) <= (int64_t)ring_len,
+                "dsx_dedisp_prime: centres [c_lo, c_lo + n_centres) are not held by the ring");
+    DSX_REQUIRE(n_dm <= 32767, "dsx_dedisp_prime: n_dm=%d > 32767", n_dm);
+    const dim3 grid((unsigned)n_pix, (unsigned)((n_dm + DD_THREADS - 1) / DD_THREADS));
+    hipStream_t s = as_stream(stream);
+    if (ring_dtype == DSX_F16)
+        dedisp_prime_kernel<_Float16><<<grid, DD_THREADS, 0, s>>>(
+            reinterpret_cast<const _Float16 *>(ring), n_sub, n_pix, hist, ring_len, T0, shift, n_dm, valid_from,
+            n_prev, c_lo, n_centres, mean);
+    else
+        dedisp_prime_kernel<float><<<grid, DD_THREADS, 0, s>>>(
+            reinterpret_cast<const float *>(ring), n_sub, n_pix, hist, ring_len, T0, shift, n_dm, valid_from,
+            n_prev, c_lo, n_centres, mean);
+    return check_launch("dsx_dedisp_prime");
 }

@@ -34,8 +34,9 @@ PROTOTYPES = {
     "dsx_ring_advance": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],
     "dsx_downsample_time": [_P, _P, _I, _I, _I64, _I, _I, _P],
     "dsx_dedisp_plan": [_P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P],
-    "dsx_dedisp_detect": [_P, _I, _I, _I, _I, _I, _I64, _P, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P,
-                          _P, _P, _I64, _I, _I, _I, _I, _I, _F, _D, _P, _I, _P, _P],
+    "dsx_dedisp_detect": [_P, _I, _I, _I, _I, _I, _I64, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P,
+                          _I, _I64, _I, _I, _I, _I, _I, _F, _D, _P, _I, _P, _P],
+    "dsx_dedisp_prime": [_P, _I, _I, _I, _I, _I, _I64, _P, _I, _P, _P, _I64, _I, _P, _P],
     "dsx_combine": [_P, _P, _P, _I, _I, _I64, _I, _I, _P],
     "dsx_synth_block": [_P, _U64, _I64, _I, _I, _I, _P, _I, _P, _P, _P, _P],
     "dsx_boxcar": [_P, _P, _I, _I, _I, _P],

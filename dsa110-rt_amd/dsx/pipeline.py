@@ -113,10 +113,11 @@ class SearchNode:
                       for _ in range(nd)]
         self.cur = 0
         ns = self.n_pix if with_detect else 0
-        # float64 noise state in n_slots + 1 rotating buffers: block k reads
-        # buffer k and writes the merge into buffer k + 1, so the state a block
-        # was detected against stays intact until block k + n_slots + 1 -- long
-        # enough to re-detect a block whose records overflowed (finish())
+        # float64 noise state in n_slots + 1 rotating buffers: block k copies
+        # buffer k into buffer k + 1 and merges its sums there in place, so the
+        # state a block was detected against stays intact until block
+        # k + n_slots + 1 -- long enough to detect a block whose records
+        # overflowed again (finish())
         self.n_stat = self.n_slots + 1
         self.mean_bufs = [torch.zeros((ns, cfg.n_dm), dtype=torch.float64, device=self.dev)
                           for _ in range(self.n_stat)]
@@ -292,13 +293,23 @@ class SearchNode:
         cand, n_cand = self.cand_slots[slot], self.n_cand_slots[slot]
         si, so = self.stat_cur, (self.stat_cur + 1) % self.n_stat
         seq = self.block_seq
+        cfg = self.cfg
         with torch.cuda.stream(s):
             self._blk_pinned[slot].copy_(torch.from_numpy(counts))
             self.blk_count.copy_(self._blk_pinned[slot], non_blocking=True)
+            self.mean_bufs[so].copy_(self.mean_bufs[si])
+            self.m2_bufs[so].copy_(self.m2_bufs[si])
+            if bool((self.n_prev_host == 0).any()):
+                # DMs without state: centre this block's sums on a few of its samples
+                _lib.call("dsx_dedisp_prime", _lib.ptr(self.rings[self.cur]),
+                          _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
+                          cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(self.shift), cfg.n_dm,
+                          _lib.ptr(self.valid_from), _lib.ptr(self.n_prev_bufs[si]), c_lo, n_steps,
+                          _lib.ptr(self.mean_bufs[so]), _lib.stream(s))
             n_cand.zero_()
             self._mark("dedisp", 0)
-            self._launch_dedisp(plan, pd, si, self.mean_bufs[so], self.m2_bufs[so], cand, n_cand, T0,
-                                n_steps, s, float(self.cfg.threshold), self.cur)
+            self._launch_dedisp(plan, pd, self.n_prev_bufs[si], self.mean_bufs[so], self.m2_bufs[so], True,
+                                cand, n_cand, T0, n_steps, s, float(self.cfg.threshold), self.cur)
             self._mark("dedisp", 1)
             torch.add(self.n_prev_bufs[si], self.blk_count, out=self.n_prev_bufs[so])
             self._cnt_pinned[slot].copy_(n_cand, non_blocking=True)
@@ -313,23 +324,21 @@ class SearchNode:
         return PendingBlock(slot=slot, T0=T0, n_steps=n_steps, done=done, stream=s, seq=seq, stat_in=si,
                             ring=self.cur, plan_key=n_steps)
 
-    def _launch_dedisp(self, plan, pd, stat_in, mean_out, m2_out, cand, n_cand, T0, n_steps, s,
+    def _launch_dedisp(self, plan, pd, n_prev, mean, m2, merge, cand, n_cand, T0, n_steps, s,
                        threshold, ring, cand_cap=None) -> None:
         """One ``dsx_dedisp_detect`` launch on history buffer ``ring`` against
-        state buffer ``stat_in``; ``mean_out`` None detects without merging,
-        ``cand_cap`` 0 only counts records."""
+        the state ``mean`` / ``m2`` behind ``n_prev`` samples, merged in place
+        when ``merge``; ``cand_cap`` 0 only counts records."""
         cfg = self.cfg
         _lib.call("dsx_dedisp_detect", _lib.ptr(self.rings[ring]),
                   _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
-                  cfg.n_sub, self.n_pix, self.hist, self.row_len, T0, _lib.ptr(self.shift),
+                  cfg.n_sub, self.n_pix, self.hist, self.row_len, T0,
                   _lib.ptr(pd["sub_info"]), _lib.ptr(pd["rows"]) if plan.n_rows else None, plan.n_rows,
                   _lib.ptr(pd["tbase"]), plan.n_terms, cfg.n_dm, _lib.ptr(self.valid_from),
-                  _lib.ptr(self.n_prev_bufs[stat_in]), _lib.ptr(self.blk_count),
-                  _lib.ptr(self.mean_bufs[stat_in]), _lib.ptr(self.m2_bufs[stat_in]),
-                  _lib.ptr(mean_out), _lib.ptr(m2_out), T0 - DETECT_DELAY_STEPS, n_steps,
-                  self.n_widths, plan.tile, plan.r_total, plan.q_total, float(threshold),
-                  float(cfg.min_stat_count), _lib.ptr(cand), self.cand_cap if cand_cap is None else int(cand_cap),
-                  _lib.ptr(n_cand), _lib.stream(s))
+                  _lib.ptr(n_prev), _lib.ptr(self.blk_count), _lib.ptr(mean), _lib.ptr(m2), int(bool(merge)),
+                  T0 - DETECT_DELAY_STEPS, n_steps, self.n_widths, plan.tile, plan.r_total, plan.q_total,
+                  float(threshold), float(cfg.min_stat_count), _lib.ptr(cand),
+                  self.cand_cap if cand_cap is None else int(cand_cap), _lib.ptr(n_cand), _lib.stream(s))
 
     AUTOTUNE_BUDGETS = (16 << 10, 24 << 10, 32 << 10, 48 << 10, 64 << 10, 80 << 10, 112 << 10, 160 << 10)
 
@@ -390,10 +399,12 @@ class SearchNode:
                 with torch.cuda.stream(s):
                     for _ in range(reps):
                         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                        mean.copy_(self.mean)
+                        m2.copy_(self.m2)
                         n_cand.zero_()
                         e0.record(s)
-                        self._launch_dedisp(plan, pd, self.stat_cur, mean, m2, cand, n_cand, self.T0, n_steps, s,
-                                            float(self.cfg.threshold), self.cur)
+                        self._launch_dedisp(plan, pd, self.n_prev, mean, m2, True, cand, n_cand, self.T0, n_steps,
+                                            s, float(self.cfg.threshold), self.cur)
                         e1.record(s)
                         e1.synchronize()
                         ms.append(e0.elapsed_time(e1))
@@ -457,8 +468,10 @@ class SearchNode:
         def probe(jj, keep):
             with torch.cuda.stream(pend.stream):
                 n_cand.zero_()
-                self._launch_dedisp(plan, pd, pend.stat_in, None, None, cand, n_cand, pend.T0, pend.n_steps,
-                                    pend.stream, float(levels[jj]), pend.ring, None if keep else 0)
+                si = pend.stat_in
+                self._launch_dedisp(plan, pd, self.n_prev_bufs[si], self.mean_bufs[si], self.m2_bufs[si], False,
+                                    cand, n_cand, pend.T0, pend.n_steps, pend.stream, float(levels[jj]), pend.ring,
+                                    None if keep else 0)
                 return int(n_cand[0].item())
 
         counts = {}

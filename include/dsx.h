@@ -169,36 +169,48 @@ int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, int n_widths
  * {int32 t - c_lo, int32 p, int16 d, int16 k, float snr}: the record's time
  * is relative to c_lo, so it fits int32 however long the stream runs) when
  * snr > threshold, counted by *n_cand (device int32 the caller zeroes;
- * entries past cand_cap are dropped but still counted, and cand_cap 0 only
- * counts).  Detection with mean_out = NULL leaves the state unchanged, so a
- * caller whose records overflowed can detect the block again at a higher
- * threshold.
- * Samples t in [c_lo, c_lo + n_centres) with t >= valid_from[d] feed the
- * block's sums, merged (Chan) with the float64 running mean / m2 ([p][d]
- * layout, n_prev[d] samples) into mean_out / m2_out (which may alias mean /
- * m2) with blk_count[d] samples; mean_out = m2_out = NULL detects without
- * merging.  Samples are accumulated in float32 centred on the running mean
- * (or, before a DM has statistics, on one of its complete samples), so the
- * sums keep their precision for pixels that hold a steady source.
- * shift is the device [s][d] int32 table of the plan.  sub_info, rows, tbase,
- * n_rows, n_terms, tile, r_total and q_total come from dsx_dedisp_plan
- * (copied to device memory; rows may be NULL when n_rows == 0; an unpaired
- * plan with q_total == r_total selects the two-buffer layout of mode 3 and
- * needs ring_dtype DSX_F32).  ring_dtype DSX_F16 or DSX_F32;
- * hist >= 64 + w_max/2 + max shift.
+ * entries past cand_cap are dropped but still counted; cand_cap 0 only
+ * counts).  When merge != 0, samples t in [c_lo, c_lo + n_centres) with
+ * t >= valid_from[d] feed the block's sums, merged (Chan) into the float64
+ * running mean / m2 ([p][d] layout, n_prev[d] samples before) in place with
+ * blk_count[d] samples; merge = 0 leaves the state unchanged, so a caller
+ * whose records overflowed can detect the block again at a higher
+ * threshold.  Samples are accumulated in float32 centred on the running
+ * mean rounded to 11 significant bits (see dsx_dedisp_prime for a DM's
+ * first block), so the sums keep their precision for pixels that hold a
+ * steady source.  sub_info, rows, tbase, n_rows, n_terms, tile, r_total
+ * and q_total come from dsx_dedisp_plan (copied to device memory; rows may
+ * be NULL when n_rows == 0; an unpaired plan with q_total == r_total selects
+ * the two-buffer layout of mode 3 and needs ring_dtype DSX_F32).
+ * ring_dtype DSX_F16 or DSX_F32; hist >= 64 + w_max/2 + max shift.
  * Replaces combine_chgroups + boxcar_via_cumsum + multi_boxcar_argmax +
  * the Layer-1/Layer-2 normalisation, src/dsart/fine_dm/combiner.py:548-655,
  * src/dsart/detector/forward.py:163-504,1264-1854,
  * src/dsart/detector/triton_boxcar.py:193-403,
  * src/dsart/noise_norm/layer2.py:313-452. */
 int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, int n_pix, int hist,
-                      int ring_len, int64_t T0, const int32_t *shift, const int32_t *sub_info,
-                      const int32_t *rows, int n_rows, const int32_t *tbase, int n_terms, int n_dm,
+                      int ring_len, int64_t T0, const int32_t *sub_info, const int32_t *rows,
+                      int n_rows, const int32_t *tbase, int n_terms, int n_dm,
                       const int32_t *valid_from, const double *n_prev, const double *blk_count,
-                      const double *mean, const double *m2, double *mean_out, double *m2_out,
-                      int64_t c_lo, int n_centres, int n_widths, int tile, int r_total, int q_total,
-                      float threshold, double min_count, void *cand, int cand_cap, int32_t *n_cand,
+                      double *mean, double *m2, int merge, int64_t c_lo, int n_centres,
+                      int n_widths, int tile, int r_total, int q_total, float threshold,
+                      double min_count, void *cand, int cand_cap, int32_t *n_cand,
                       dsx_stream_t stream);
+
+/* Centre of each DM's first block: for every pixel p and DM d with
+ * n_prev[d] == 0, mean[p][d] = the float64 average of its complete samples
+ * D[d][t][p], max(valid_from[d], c_lo) <= t < c_lo + n_centres, from
+ * per-subband prefix sums of the ring (same layout as dsx_dedisp_detect;
+ * shift is the device [s][d] int32 table; the ring must hold those
+ * samples: hist + c_lo - T0 >= max shift).  dsx_dedisp_detect then centres
+ * the block's float32 sums on it; with n_prev[d] = 0 the merge gives it no
+ * weight.  DMs with state are untouched.
+ * Part of the Layer-2 burn-in the north-star statistics replace,
+ * src/dsart/noise_norm/layer2.py:313-452 (Welford burn-in). */
+int dsx_dedisp_prime(const void *ring, int ring_dtype, int n_sub, int n_pix, int hist, int ring_len,
+                     int64_t T0, const int32_t *shift, int n_dm, const int32_t *valid_from,
+                     const double *n_prev, int64_t c_lo, int n_centres, double *mean,
+                     dsx_stream_t stream);
 
 /* Shift-and-add combine of n_streams float32 streams (n_streams, t_stream,
  * n_inner): out[t][k] = sum_g streams[g][t0 + t - shift[g]][k] for

@@ -44,23 +44,22 @@ def test_version_and_error_paths():
     p = ctypes.cast(buf, ctypes.c_void_p)
     rc = L.dsx_corr_i8(p, None, p, 4, 2, 1, 3, 1, 1, 0, 1, None)     # odd t_int
     assert rc == -1 and b"t_int" in L.dsx_last_error()
-    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, p, None, 0, p, 1, 1, p, p, p, p, p, p, p, 0, 65, 1, 64,
-                             10, 0, ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
+    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, p, p, 1, 0, 65, 1, 64, 10, 0,
+                             ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
     assert rc == -1 and b"n_centres" in L.dsx_last_error()
-    # merged state: both outputs or neither
-    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, p, None, 0, p, 1, 1, p, p, p, p, p, p, None, 0, 64, 1, 64,
-                             10, 0, ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
-    assert rc == -1 and b"both" in L.dsx_last_error()
-    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, None, p, None, 0, p, 1, 1, p, p, p, p, p, p, p, 0, 64, 1, 64,
-                             10, 0, ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
+    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, None, p, 1, 0, 64, 1, 64, 10, 0,
+                             ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
     assert rc == -1 and b"null" in L.dsx_last_error()
+    # the prime kernel's centres must be held by the ring
+    rc = L.dsx_dedisp_prime(p, 0, 1, 1, 10, 20, 0, p, 1, p, p, -64, 64, p, None)
+    assert rc == -1 and b"not held" in L.dsx_last_error()
     rc = L.dsx_image_subbands(p, p, p, p, p, 1, 1, 1, 1, 100, 1, 2, ctypes.c_float(1.0), 1, None)
     assert rc == -1 and b"power of two" in L.dsx_last_error()
     rc = L.dsx_ring_advance(p, p, p, 1, 1, 1, 1, 0, 1, 1, None)
     assert rc == -1 and b"differ" in L.dsx_last_error()
     # two window buffers (q_total == r_total on an unpaired plan) need a float32 history (ring_dtype 1 = fp16)
-    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, p, None, 0, p, 1, 1, p, p, p, p, p, p, p, 0, 64, 1, 64,
-                             10, 10, ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
+    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, p, p, 1, 0, 64, 1, 64, 10, 10,
+                             ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
     assert rc == -1 and b"describe a plan" in L.dsx_last_error()
     buf2 = ctypes.create_string_buffer(64)
     p2 = ctypes.cast(buf2, ctypes.c_void_p)

@@ -184,3 +184,48 @@ def test_ring_advance_matches_numpy(dtype, hist):
         torch.cuda.synchronize()
         got = rings[cur].float().cpu().numpy()
         assert np.array_equal(got[:, :, :hist + n_new], want[:, :, :hist + n_new]), f"block of {n_new} steps"
+
+
+@pytest.mark.parametrize("dtype,row_len", [(torch.float32, 2300), (torch.float16, 4500)])
+def test_dedisp_prime_first_block_means(dtype, row_len):
+    """dsx_dedisp_prime: for DMs without state the written mean is the float64
+    average of the DM's complete samples of the block (summed here by brute
+    force); DMs with state keep their mean.  Rows longer than one LDS chunk
+    (4500 > 2048) and shifts reaching back to position 0 are covered."""
+    from dsx import _lib
+    rng = np.random.default_rng(8)
+    n_sub, n_pix, n_dm, hist = 5, 6, 300, 200
+    T0 = 10_000
+    L = row_len - hist
+    c_lo, n_c = T0 - 64, L
+    sh = rng.integers(0, hist - 64 + 1, size=(n_sub, n_dm)).astype(np.int32)
+    sh[:, 0] = 0
+    sh[0, 1] = hist - 64
+    vf = sh.max(axis=0).astype(np.int32)
+    vf[5] = T0 + 100                     # complete only part way into the block
+    vf[6] = T0 + L                       # no complete sample in this block
+    n_prev = np.zeros(n_dm)
+    n_prev[::7] = 50.0                   # these DMs have state
+    ring = (rng.standard_normal((n_sub, n_pix, row_len)) + 3.0).astype(np.float32)
+    ring_t = torch.from_numpy(ring).to(dtype).cuda().contiguous()
+    ring = ring_t.float().cpu().numpy().astype(np.float64)
+    mean0 = rng.standard_normal((n_pix, n_dm))
+    mean_t = torch.from_numpy(mean0.copy()).cuda()
+    sh_t, vf_t, np_t = (torch.from_numpy(x).cuda() for x in (sh, vf, n_prev))     # alive until the kernel ran
+    _lib.call("dsx_dedisp_prime", _lib.ptr(ring_t), _lib.DSX_F16 if dtype == torch.float16 else _lib.DSX_F32,
+              n_sub, n_pix, hist, row_len, T0, _lib.ptr(sh_t), n_dm, _lib.ptr(vf_t), _lib.ptr(np_t), c_lo, n_c,
+              _lib.ptr(mean_t), None)
+    torch.cuda.synchronize()
+    got = mean_t.cpu().numpy()
+    want = mean0.copy()
+    for d in range(n_dm):
+        if n_prev[d] != 0:
+            continue
+        t_b = max(int(vf[d]), c_lo)
+        if t_b >= c_lo + n_c:
+            continue
+        t = np.arange(t_b, c_lo + n_c)
+        D = sum(ring[s][:, hist + t - sh[s, d] - T0] for s in range(n_sub))      # (n_pix, n_t)
+        want[:, d] = D.mean(axis=1)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(got[:, 6], mean0[:, 6]) and np.array_equal(got[:, ::7], mean0[:, ::7])

@@ -11,7 +11,7 @@ The images carry per-pixel offsets of up to 8 sigma in every subband
 statistics and boxcars is exercised, and a dispersed pulse at DM index 200
 in blocks 2 and 3.  Bars: chunk-maximum records identical in (DM, time,
 pixel, width) with S/N within 1e-4 relative; float64 mean / M2 within 1e-5
-relative (the mean also within 1e-6 of the dedispersed series' sigma)."""
+relative (the mean also within 2e-6 of the dedispersed series' sigma)."""
 import numpy as np
 import pytest
 import torch
@@ -78,7 +78,7 @@ def test_node_plan_every_layout_vs_oracle(ring_dtype):
             nd.push_images(img_t, L)
             res = nd.detect(L)
             k = compare_records(gpu_records(res.records), want, cfg.threshold, 1e-4, f"block {blk} {name}")
-            np.testing.assert_allclose(nd.mean.cpu().numpy().T, o.st_mean, rtol=1e-5, atol=1e-6 * np.sqrt(cfg.n_sub),
+            np.testing.assert_allclose(nd.mean.cpu().numpy().T, o.st_mean, rtol=1e-5, atol=2e-6 * np.sqrt(cfg.n_sub),
                                        err_msg=f"block {blk} {name} mean")
             np.testing.assert_allclose(nd.m2.cpu().numpy().T, o.st_m2, rtol=1e-5, atol=1e-5,
                                        err_msg=f"block {blk} {name} m2")

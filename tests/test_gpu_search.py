@@ -5,8 +5,10 @@ Bars: chunk-maximum records identical in (DM, chunk, pixel, time, width)
 with S/N within 1e-4 relative for float32 images; decoded candidate keys
 (DM index, time bin, width, row, col) identical with S/N within 2e-3
 relative; float64 statistics within 1e-5 relative (the mean, which sits near
-zero for noise, within 1e-5 relative or 1e-6 of the dedispersed series'
-sigma, sqrt(n_sub) for unit-variance images).
+zero for noise, within 1e-5 relative or 2e-6 of the dedispersed series'
+sigma, sqrt(n_sub) for unit-variance images: the kernel sums float32 samples
+centred on the running mean, and a DM whose first block held only a few
+complete samples starts from a rough mean).
 """
 import numpy as np
 import pytest
@@ -101,7 +103,7 @@ def test_dedisp_detect_random_images_vs_oracle(layout):
         total += len(g)
         mean = node.mean.cpu().numpy().T
         m2 = node.m2.cpu().numpy().T
-        np.testing.assert_allclose(mean, o.st_mean, rtol=1e-5, atol=1e-6 * np.sqrt(cfg.n_sub))
+        np.testing.assert_allclose(mean, o.st_mean, rtol=1e-5, atol=2e-6 * np.sqrt(cfg.n_sub))
         np.testing.assert_allclose(m2, o.st_m2, rtol=1e-5, atol=1e-5)
     assert total > 0
 
@@ -140,7 +142,7 @@ def test_dedisp_detect_ragged_shapes_vs_oracle(mode):
         g = _gpu_records(res.records)
         _compare_records(g, _oracle_records(o, cfg.threshold), cfg.threshold, 1e-4)
         total += len(g)
-        np.testing.assert_allclose(node.mean.cpu().numpy().T, o.st_mean, rtol=1e-5, atol=1e-6 * np.sqrt(cfg.n_sub))
+        np.testing.assert_allclose(node.mean.cpu().numpy().T, o.st_mean, rtol=1e-5, atol=2e-6 * np.sqrt(cfg.n_sub))
         np.testing.assert_allclose(node.m2.cpu().numpy().T, o.st_m2, rtol=1e-5, atol=1e-5)
     assert any(k[0] >= 256 for k in g), "no record from the second DM work-group"
     assert total > 0
@@ -200,7 +202,7 @@ def test_steady_sources_keep_float64_statistics(mode):
         g = _gpu_records(res.records)
         _compare_records(g, _oracle_records(o, cfg.threshold), cfg.threshold, 1e-4)
         total += len(g)
-        np.testing.assert_allclose(node.mean.cpu().numpy().T, o.st_mean, rtol=1e-5, atol=1e-6 * np.sqrt(cfg.n_sub))
+        np.testing.assert_allclose(node.mean.cpu().numpy().T, o.st_mean, rtol=1e-5, atol=2e-6 * np.sqrt(cfg.n_sub))
         np.testing.assert_allclose(node.m2.cpu().numpy().T, o.st_m2, rtol=1e-5, atol=1e-5)
     assert total > 0
 

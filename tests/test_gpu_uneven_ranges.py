@@ -7,7 +7,7 @@ full block per range (2048 / 1024 / 512 steps) for four blocks of random
 images on a 16 x 16 grid, under every LDS layout the planner offers.
 
 Bars: chunk-maximum records identical in (DM, time, pixel, width) with S/N
-within 1e-4 relative; float64 mean / M2 within 1e-5 relative (the mean also within 1e-6 of the
+within 1e-4 relative; float64 mean / M2 within 1e-5 relative (the mean also within 2e-6 of the
 dedispersed series' sigma)."""
 import numpy as np
 import pytest
@@ -62,7 +62,7 @@ def test_uneven_range_every_layout_vs_oracle(rng_idx):
             nd.push_images(img_t, L)
             res = nd.detect(L)
             total += compare_records(gpu_records(res.records), want, rc.threshold, 1e-4, f"block {blk} {name}")
-            np.testing.assert_allclose(nd.mean.cpu().numpy().T, o.st_mean, rtol=1e-5, atol=1e-6 * np.sqrt(rc.n_sub))
+            np.testing.assert_allclose(nd.mean.cpu().numpy().T, o.st_mean, rtol=1e-5, atol=2e-6 * np.sqrt(rc.n_sub))
             np.testing.assert_allclose(nd.m2.cpu().numpy().T, o.st_m2, rtol=1e-5, atol=1e-5)
         print(f"block {blk}: {len(want)} oracle records", flush=True)
     assert total > 0
