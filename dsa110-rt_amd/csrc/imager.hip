@@ -81,8 +81,27 @@ struct ImgParams {
     void *img;
     int n_steps, n_sub, per_sub, nbase, N, log2N, n_rows, rc_max, npad;
     float scale;
-    int dense;                // 0: taps from int32 visibilities (+ mask); 1: dense grid, no mask
+    int dense;                // 0: taps from int32 visibilities (+ mask); 1: dense grid, no mask;
+                              // 2: shift-and-add of cint8 uv streams (cube mode, below)
+    // cube mode: item (f, t), t in [t_lo, t_lo + n_steps), f < n_sub:
+    //   G[x][y] = sum_g cal_g(streams[g][t - shifts[f][g]][re, im][x][y]) for in-range rows,
+    //   img[t][f] = mask * fftshift(Re(ifft2(scale * G))) into a [T][n_sub][N][N] cube
+    const int8_t *streams;    // [n_chg][t_stream][2][N][N]
+    const int32_t *shifts;    // [n_sub][n_chg] (device)
+    const float *cal_scale;   // [n_chg] or [n_chg][t_stream] (cal_2d), NULL: unit scale
+    const float *cal_off_re;  // same shapes, NULL: zero
+    const float *cal_off_im;
+    const float *mask;        // [N][N] (or [n_sub][N][N], mask_stride N^2) multiplied in the epilogue
+    float *acc;               // float32 partial images when the output type is not float32
+    int n_chg, t_stream, t_lo, cal_2d;
+    int64_t mask_stride;      // 0: one mask for every item; N^2: one per f
 };
+
+// cint8 cube streams: the calibrated value of one (chgroup, row, cell)
+__device__ __forceinline__ float cal_at(const float *a, int cal_2d, int g, int t_stream, int tau, float dflt) {
+    if (a == nullptr) return dflt;
+    return cal_2d ? a[(int64_t)g * t_stream + tau] : a[g];
+}
 
 // R16: N = 256 row transforms as two in-register 16-point passes (rows padded
 // to RS = 272 entries so the pass-1 transposed writes are conflict-free);
@@ -118,14 +137,26 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
     const int n_chunks = (P.n_rows + P.rc_max - 1) / P.rc_max;
     TOut *img = reinterpret_cast<TOut *>(P.img);
 
+    // cube mode: per-item stream rows (-1: outside [0, t_stream)) after the H rows
+    int *tau_s = reinterpret_cast<int *>(smem + N + (R16 ? 256 : 0) + P.rc_max * RS);
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
         const int s = item / P.n_steps;
         const int t = item - s * P.n_steps;
         const int32_t *vslab = P.dense ? nullptr
                                        : P.vis + 2 * ((int64_t)t * n_cout + (int64_t)s * P.per_sub) * P.nbase;
         const int tap_lo = P.dense ? 0 : P.tap_off[s], tap_hi = P.dense ? 0 : P.tap_off[s + 1];
-        const float2 *gsrc = P.dense ? P.grid + (int64_t)item * N * N : nullptr;
-        TOut *out = img + ((int64_t)s * P.n_steps + t) * N * N;
+        const float2 *gsrc = P.dense == 1 ? P.grid + (int64_t)item * N * N : nullptr;
+        const int64_t o_item = P.dense == 2 ? ((int64_t)(P.t_lo + t) * P.n_sub + s) * N * N
+                                            : ((int64_t)s * P.n_steps + t) * N * N;
+        TOut *out = img + o_item;
+        float *acc_out = P.acc != nullptr ? P.acc + o_item : nullptr;
+        if (P.dense == 2) {
+            __syncthreads();                 // the previous item's rows are no longer read
+            for (int g = tid; g < P.n_chg; g += IMG_THREADS) {
+                const int tau = P.t_lo + t - P.shifts[(int64_t)s * P.n_chg + g];
+                tau_s[g] = (tau >= 0 && tau < P.t_stream) ? tau : -1;
+            }
+        }
         for (int ck = 0; ck < n_chunks; ++ck) {
             const int r0 = ck * P.rc_max;
             const int rc = min(P.rc_max, P.n_rows - r0);
@@ -136,8 +167,45 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                 for (int i = tid; i < rc * N; i += IMG_THREADS) {
                     const int r = i >> P.log2N, w = i & (N - 1);
                     const int rg = r0 + r;
-                    const float2 a = gsrc[(int64_t)rg * N + w];
-                    const float2 b = gsrc[(int64_t)((N - rg) & (N - 1)) * N + ((N - w) & (N - 1))];
+                    const int64_t ca = (int64_t)rg * N + w;
+                    const int64_t cb = (int64_t)((N - rg) & (N - 1)) * N + ((N - w) & (N - 1));
+                    float2 a, b;
+                    if (P.dense == 2) {
+                        // shift-and-add of the chgroups' cint8 rows at this cell and its mirror
+                        const int64_t plane = (int64_t)N * N;
+                        if (P.cal_scale == nullptr && P.cal_off_re == nullptr && P.cal_off_im == nullptr) {
+                            int ar = 0, ai = 0, br = 0, bi = 0;      // exact integer sums
+                            for (int g = 0; g < P.n_chg; ++g) {
+                                const int tau = tau_s[g];
+                                if (tau < 0) continue;
+                                const int8_t *row = P.streams + ((int64_t)g * P.t_stream + tau) * 2 * plane;
+                                ar += row[ca];
+                                ai += row[plane + ca];
+                                br += row[cb];
+                                bi += row[plane + cb];
+                            }
+                            a = make_float2((float)ar, (float)ai);
+                            b = make_float2((float)br, (float)bi);
+                        } else {
+                            a = make_float2(0.f, 0.f);
+                            b = make_float2(0.f, 0.f);
+                            for (int g = 0; g < P.n_chg; ++g) {
+                                const int tau = tau_s[g];
+                                if (tau < 0) continue;
+                                const int8_t *row = P.streams + ((int64_t)g * P.t_stream + tau) * 2 * plane;
+                                const float sc = cal_at(P.cal_scale, P.cal_2d, g, P.t_stream, tau, 1.f);
+                                const float ore = cal_at(P.cal_off_re, P.cal_2d, g, P.t_stream, tau, 0.f);
+                                const float oim = cal_at(P.cal_off_im, P.cal_2d, g, P.t_stream, tau, 0.f);
+                                a.x += fmaf(sc, (float)row[ca], ore);
+                                a.y += fmaf(sc, (float)row[plane + ca], oim);
+                                b.x += fmaf(sc, (float)row[cb], ore);
+                                b.y += fmaf(sc, (float)row[plane + cb], oim);
+                            }
+                        }
+                    } else {
+                        a = gsrc[ca];
+                        b = gsrc[cb];
+                    }
                     const float sg = ((rg + w) & 1) ? -1.f : 1.f;
                     const int wr = R16 ? w : (int)(__brev((unsigned)w) >> (32 - P.log2N));
                     H[r * RS + wr] = make_float2(sg * (a.x + b.x), sg * (a.y - b.y));
@@ -322,10 +390,18 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                         const int i = i0 + 4 * lg + v;
                         const bool inside = P.dense || (i >= P.npad && i < N - P.npad && j >= P.npad &&
                                                         j < N - P.npad && !(i == halfN && j == halfN));
-                        float val = inside ? acc[v] * out_scale : 0.f;
                         const int64_t o = (int64_t)i * N + j;
-                        if (ck > 0) val += ld_as_float(out, o);
-                        st_from_float(out, o, val);
+                        float val = inside ? acc[v] * out_scale : 0.f;
+                        if (P.mask != nullptr) val *= P.mask[(int64_t)s * P.mask_stride + o];
+                        if (acc_out != nullptr) {
+                            // partial images in float32; the last pass writes the output type
+                            if (ck > 0) val += acc_out[o];
+                            if (ck + 1 < n_chunks) acc_out[o] = val;
+                            else st_from_float(out, o, val);
+                        } else {
+                            if (ck > 0) val += ld_as_float(out, o);
+                            st_from_float(out, o, val);
+                        }
                     }
                     }
                 }
@@ -365,9 +441,9 @@ static int img_rows_per_chunk(int n_grid, bool r16) {
     return std::min(rc, 2 * IMG_MAX_KB);
 }
 
-static size_t img_lds_bytes(int n_grid, int rc_max, bool r16) {
+static size_t img_lds_bytes(int n_grid, int rc_max, bool r16, int n_chg = 0) {
     const size_t rs = r16 ? R16_RS : n_grid;
-    return ((size_t)n_grid + (r16 ? 256 : 0) + (size_t)rc_max * rs) * sizeof(float2);
+    return ((size_t)n_grid + (r16 ? 256 : 0) + (size_t)rc_max * rs) * sizeof(float2) + (size_t)n_chg * sizeof(int);
 }
 
 template <typename TOut, bool R16>
@@ -454,4 +530,60 @@ extern "C" int dsx_dirty_image(const void *grid, void *img, int n_images, int n_
                        : launch_image<float, false>(P, blocks, lds, as_stream(stream));
     if (rc != DSX_OK) return rc;
     return DSX_OK;
+}
+
+extern "C" int dsx_image_cube(const int8_t *streams, const int32_t *shifts, int n_chg, int t_stream,
+                              int n_fdm, int t_det, int t_lo, int n_grid, float prescale,
+                              const float *cal_scale, const float *cal_off_re, const float *cal_off_im,
+                              int cal_2d, const float *mask, int mask_per_fdm, void *cube, int cube_dtype,
+                              float *acc, dsx_stream_t stream) {
+    DSX_REQUIRE(streams && shifts && cube, "dsx_image_cube: null pointer");
+    DSX_REQUIRE(n_chg > 0 && n_chg <= 4096 && t_stream > 0 && n_fdm > 0 && t_det > 0,
+                "dsx_image_cube: bad shape");
+    DSX_REQUIRE(t_lo >= 0 && t_lo < t_det, "dsx_image_cube: t_lo=%d not in [0, t_det=%d)", t_lo, t_det);
+    DSX_REQUIRE(n_grid >= 16 && n_grid <= 1024 && (n_grid & (n_grid - 1)) == 0,
+                "dsx_image_cube: n_grid must be a power of two in [16, 1024]");
+    DSX_REQUIRE(cube_dtype == DSX_F16 || cube_dtype == DSX_F32, "dsx_image_cube: cube_dtype");
+    DSX_REQUIRE((int64_t)n_fdm * (t_det - t_lo) <= INT32_MAX, "dsx_image_cube: too many images");
+    int log2N = 0;
+    while ((1 << log2N) < n_grid) ++log2N;
+    const int n_rows = n_grid / 2 + 1;
+    const bool r16 = n_grid == 256;
+    int rc_max = std::min(img_rows_per_chunk(n_grid, r16), n_rows);
+    if (rc_max < 1) return set_error(DSX_ERANGE, "dsx_image_cube: n_grid too large");
+    const int n_chunks = (n_rows + rc_max - 1) / rc_max;
+    if (cube_dtype != DSX_F32 && n_chunks > 1 && acc == nullptr)
+        return set_error(DSX_EINVAL, "dsx_image_cube: a float16 cube needs a float32 acc buffer "
+                                     "(%d uv row passes)", n_chunks);
+    ImgParams P{};
+    P.img = cube;
+    P.n_steps = t_det - t_lo;
+    P.n_sub = n_fdm;
+    P.per_sub = 1;
+    P.nbase = 1;
+    P.N = n_grid;
+    P.log2N = log2N;
+    P.n_rows = n_rows;
+    P.rc_max = rc_max;
+    P.scale = prescale;
+    P.dense = 2;
+    P.streams = streams;
+    P.shifts = shifts;
+    P.cal_scale = cal_scale;
+    P.cal_off_re = cal_off_re;
+    P.cal_off_im = cal_off_im;
+    P.mask = mask;
+    P.mask_stride = mask_per_fdm ? (int64_t)n_grid * n_grid : 0;
+    P.acc = cube_dtype == DSX_F32 ? nullptr : acc;
+    P.n_chg = n_chg;
+    P.t_stream = t_stream;
+    P.t_lo = t_lo;
+    P.cal_2d = cal_2d != 0;
+    const size_t lds = img_lds_bytes(n_grid, rc_max, r16, n_chg);
+    if (lds > 160 * 1024) return set_error(DSX_ERANGE, "dsx_image_cube: %zu B of LDS", lds);
+    const int blocks = (int)std::min<int64_t>((int64_t)n_fdm * (t_det - t_lo), 4096);
+    hipStream_t s = as_stream(stream);
+    if (cube_dtype == DSX_F16)
+        return r16 ? launch_image<_Float16, true>(P, blocks, lds, s) : launch_image<_Float16, false>(P, blocks, lds, s);
+    return r16 ? launch_image<float, true>(P, blocks, lds, s) : launch_image<float, false>(P, blocks, lds, s);
 }

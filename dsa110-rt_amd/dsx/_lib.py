@@ -31,6 +31,7 @@ PROTOTYPES = {
     "dsx_grid_gather": [_P, _P, _P, _P, _I, _I, _I, _I, _P],
     "dsx_image_subbands": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _F, _I, _P],
     "dsx_dirty_image": [_P, _P, _I, _I, _F, _P],
+    "dsx_image_cube": [_P, _P, _I, _I, _I, _I, _I, _I, _F, _P, _P, _P, _I, _P, _I, _P, _I, _P, _P],
     "dsx_ring_advance": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],
     "dsx_downsample_time": [_P, _P, _I, _I, _I64, _I, _I, _P],
     "dsx_dedisp_plan": [_P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P],
@@ -38,6 +39,9 @@ PROTOTYPES = {
                           _I, _I64, _I, _I, _I, _I, _I, _F, _D, _P, _I, _P, _P],
     "dsx_dedisp_prime": [_P, _I, _I, _I, _I, _I, _I64, _P, _I, _P, _P, _I64, _I, _P, _P],
     "dsx_combine": [_P, _P, _P, _I, _I, _I64, _I, _I, _P],
+    "dsx_cube_cumsum": [_P, _I, _P, _I, _I64, _I, _P],
+    "dsx_cube_candidates": [_P, _I, _I64, _I, _P, _P, _I, _F, _I, _I, _I, _P, _P, _P, _P, _I64, _P],
+    "dsx_cube_box_samples": [_P, _I, _I64, _I, _P, _I, _I, _I, _P, _I, _P, _P],
     "dsx_synth_block": [_P, _U64, _I64, _I, _I, _I, _P, _I, _P, _P, _P, _P],
     "dsx_boxcar": [_P, _P, _I, _I, _I, _P],
     "dsx_multi_boxcar_argmax": [_P, _I, _P, _P, _P, _I, _P, _I, _P, _I, _I, _I, _I, _P],

@@ -13,6 +13,7 @@
 from __future__ import annotations
 
 import enum
+import math
 from dataclasses import dataclass
 from typing import List, Optional, Sequence, Tuple
 
@@ -183,3 +184,326 @@ def multi_boxcar_argmax(cs: torch.Tensor, *, widths: torch.Tensor, offsets: torc
               _lib.DSX_F16 if eff == torch.float16 else _lib.DSX_F32, _lib.ptr(out_win), tpad,
               int(n_out), inner, int(t_base), _lib.stream())
     return out_max, out_win
+
+
+# ---------------------------------------------------------------------------
+# Detector over a materialised cube (the Detector protocol drop-in)
+# ---------------------------------------------------------------------------
+
+DETECTOR_K_TIME_WIDTHS = (1, 2, 4, 8, 16, 32, 64, 128)
+DEFAULT_MERGE_RADIUS_LM = 3
+DEFAULT_MERGE_RADIUS_FDM = 5
+DEFAULT_MERGE_RADIUS_T = 128
+CUBE_CADENCE_S_DEFAULT = 256 * 524.288e-6
+
+#: 16-byte cube candidate record of ``dsx_cube_candidates``
+CUBE_REC_DTYPE = np.dtype([("t", "<i4"), ("cell", "<i4"), ("k", "<i4"), ("snr", "<f4")])
+
+
+@dataclass(frozen=True)
+class Kernel:
+    """One detector kernel (the v1 bank: unit image kernel, unit DM width,
+    a time boxcar), ``src/dsart/detector/forward.py`` kernel records."""
+    kernel_id: str
+    k_time_width: int
+    k_dm_width: int = 1
+    image_kernel_size: int = 1
+
+
+class Layer2State:
+    """Per-kernel noise sigma_k: running mean over the first ``n_burnin``
+    cubes, then an EMA with ``gamma = 1 - exp(-cadence / tau)``; zero
+    estimates keep the old value; optional ceiling ``sigma_max_ratio * s_k``
+    with the clamp-escape hatch and a floor; invalid cubes leave the state
+    alone (``src/dsart/noise_norm/layer2.py:190-452``).  Host float64."""
+
+    def __init__(self, n_kernels: int, *, cube_cadence_s: float = CUBE_CADENCE_S_DEFAULT, tau_s: float = 30.0,
+                 n_burnin: int = 30, sigma_floor: float = 0.0, sigma_max_ratio: float = 0.0,
+                 clamp_escape_cubes: int = 0):
+        if n_kernels < 1:
+            raise ValueError(f"n_kernels={n_kernels}, expected >= 1")
+        if tau_s <= 0:
+            raise ValueError(f"tau_s={tau_s}, expected > 0")
+        if cube_cadence_s <= 0:
+            raise ValueError(f"cube_cadence_s={cube_cadence_s}, expected > 0")
+        if n_burnin < 0:
+            raise ValueError(f"n_burnin={n_burnin}, expected >= 0")
+        self.n_kernels = int(n_kernels)
+        self.gamma = 1.0 - math.exp(-float(cube_cadence_s) / float(tau_s))
+        self.n_burnin = int(n_burnin)
+        self.sigma_floor = float(sigma_floor)
+        self.sigma_max_ratio = float(sigma_max_ratio)
+        self.clamp_escape_cubes = int(clamp_escape_cubes)
+        self._s_k = np.ones(self.n_kernels)
+        self._streak = np.zeros(self.n_kernels, dtype=np.int64)
+        self._cube_count = 0
+
+    @property
+    def s_k(self) -> np.ndarray:
+        return self._s_k.copy()
+
+    @property
+    def cube_count(self) -> int:
+        return self._cube_count
+
+    @property
+    def is_warming_up(self) -> bool:
+        return self._cube_count < self.n_burnin
+
+    def seed(self, s_k) -> None:
+        self._s_k = np.asarray(s_k, dtype=np.float64).reshape(self.n_kernels).copy()
+
+    def reset(self) -> None:
+        self._s_k = np.ones(self.n_kernels)
+        self._streak[:] = 0
+        self._cube_count = 0
+
+    def update_and_query(self, *, per_kernel_sigma, valid: bool = True) -> Tuple[np.ndarray, bool]:
+        if not valid:
+            return self.s_k, self.is_warming_up
+        x = np.asarray(per_kernel_sigma, dtype=np.float64).reshape(-1).copy()
+        if x.shape != (self.n_kernels,):
+            raise ValueError(f"per-kernel sigma shape {x.shape} != (n_kernels={self.n_kernels},)")
+        x = np.where(x == 0, self._s_k, x)
+        if self.sigma_max_ratio > 0 and self._cube_count > 0:
+            high = x > self.sigma_max_ratio * self._s_k
+            if high.any():
+                esc = (high & (self._streak >= self.clamp_escape_cubes)) if self.clamp_escape_cubes > 0 \
+                    else np.zeros_like(high)
+                clamp = high & ~esc
+                self._s_k = np.where(esc, x, self._s_k)
+                x = np.where(clamp, self._s_k, x)
+                self._streak = np.where(clamp, self._streak + 1, 0)
+            else:
+                self._streak[:] = 0
+        if self._cube_count < self.n_burnin:
+            c = float(self._cube_count)
+            self._s_k = (c * self._s_k + x) / (c + 1.0)
+        else:
+            self._s_k = self.gamma * x + (1.0 - self.gamma) * self._s_k
+        if self.sigma_floor > 0:
+            self._s_k = np.maximum(self._s_k, self.sigma_floor)
+        self._cube_count += 1
+        return self.s_k, self.is_warming_up
+
+
+def sigma_clipped_std_host(samples: np.ndarray, n_sigma: float = 3.0, n_iterations: int = 3) -> float:
+    """sigma-clipped standard deviation of one sample (layer1.py:53-122):
+    clip about the lower median (``torch.nanmedian``), ``n_iterations``
+    passes at ``n_sigma``; NaN samples are ignored.  Host code on the
+    gathered sample (at most ``sigma_max_samples`` values per kernel)."""
+    s = np.asarray(samples, dtype=np.float64).reshape(-1)
+    active = np.isfinite(s)
+
+    def stats(act):
+        v = np.sort(s[act])
+        if v.size == 0:
+            return float("nan"), float("nan")
+        med = float(v[(v.size - 1) // 2])
+        return med, math.sqrt(float(np.sum((v - med) ** 2)) / v.size)
+
+    med, sig = stats(active)
+    for _ in range(int(n_iterations)):
+        if not np.isfinite(sig):
+            break
+        active = active & (s >= med - n_sigma * sig) & (s <= med + n_sigma * sig)
+        med, sig = stats(active)
+    return 0.0 if not np.isfinite(sig) else sig
+
+
+class DeterministicDetector:
+    """Detector protocol drop-in (``src/dsart/detector/forward.py:117-155``,
+    ``DeterministicDetector.forward`` ``:891-1009`` in its streaming
+    top-k form ``:1264-1854``) over a materialised ``[T_det, N_fdm, H, W]``
+    cube on the GPU.
+
+    Per cube: the padded time prefix of the cube (``dsx_cube_cumsum``);
+    centred boxcars of the bank's time widths scored against the Layer-2
+    sigma_k of *earlier* cubes, the maximum over kernels per cell over the
+    interior centres ``[w_max/2, T - w_max/2)``, and the ``n_top`` strongest
+    cells above threshold (``dsx_cube_candidates``: queue + radix select on
+    the GPU); then on the host the merge-radius suppression of
+    ``decode_topk_argmax_lowmem`` (``decoder.py:388-533``), the
+    ``NOISE_WARMUP`` flag during burn-in and the canonical fine-DM range
+    gate.  sigma_k is updated at the end of the cube (when at least
+    ``layer2_valid_min_fraction`` of the validity mask is True) from the
+    sigma-clipped std of each kernel's raw box sums at a fixed seeded sample
+    of interior cells (``dsx_cube_box_samples``), the reference's
+    ``layer2_sigma_max_samples`` path.  ``sigma_layer1`` is accepted and,
+    as in v1, unused."""
+
+    def __init__(self, *, time_widths: Sequence[int] = DETECTOR_K_TIME_WIDTHS, threshold_sigma: float = 8.0,
+                 detector_version: str = "v1.M5", device=None, merge_radius_lm: int = DEFAULT_MERGE_RADIUS_LM,
+                 merge_radius_fdm: int = DEFAULT_MERGE_RADIUS_FDM, merge_radius_t: int = DEFAULT_MERGE_RADIUS_T,
+                 search_node_id: int = 0, gpu_half: int = 0, cube_cadence_s: float = CUBE_CADENCE_S_DEFAULT,
+                 layer2_tau_s: float = 30.0, layer2_n_burnin: int = 30, n_kernel_max_t: Optional[int] = None,
+                 layer2_seed_unit: bool = True, layer2_sigma_floor: float = 0.0,
+                 layer2_sigma_max_ratio: float = 0.0, layer2_clamp_escape_cubes: int = 0,
+                 layer2_valid_min_fraction: float = 1.0, layer2_sigma_max_samples: int = 1 << 16,
+                 streaming_decoder_n_top: int = 64, spatial_active: Optional[np.ndarray] = None,
+                 queue_records: int = 1 << 20):
+        ws = [int(w) for w in time_widths]
+        if not ws or any(w < 1 for w in ws) or len(ws) > 16:
+            raise ValueError(f"time_widths={tuple(ws)}: 1-16 positive widths")
+        self._kernel_bank = tuple(Kernel(kernel_id=f"unit:d1:b{w}", k_time_width=w) for w in ws)
+        self._widths = np.asarray(ws, dtype=np.int32)
+        self._threshold = float(threshold_sigma)
+        self._detector_version = detector_version
+        self._device = torch.device(device) if device is not None else torch.device("cuda")
+        self._radii = (int(merge_radius_t), int(merge_radius_fdm), int(merge_radius_lm))
+        if min(self._radii) < 0:
+            raise ValueError(f"merge radii must be >= 0; got {self._radii}")
+        self._search_node_id = int(search_node_id)
+        self._gpu_half = int(gpu_half)
+        self._n_kernel_max_t = int(n_kernel_max_t) if n_kernel_max_t is not None else max(ws)
+        self._layer2 = Layer2State(len(ws), cube_cadence_s=cube_cadence_s, tau_s=layer2_tau_s,
+                                   n_burnin=layer2_n_burnin, sigma_floor=layer2_sigma_floor,
+                                   sigma_max_ratio=layer2_sigma_max_ratio,
+                                   clamp_escape_cubes=layer2_clamp_escape_cubes)
+        if layer2_seed_unit:
+            self._layer2.seed([math.sqrt(k.k_dm_width * k.k_time_width) for k in self._kernel_bank])
+        self._valid_min_fraction = float(layer2_valid_min_fraction)
+        self._n_samples = int(layer2_sigma_max_samples)
+        self._n_top = int(streaming_decoder_n_top)
+        self._spatial_active = None if spatial_active is None else np.asarray(spatial_active, dtype=bool).reshape(-1)
+        self._queue_records = int(queue_records)
+        self._ws = None
+        self._cs = None
+        self._samples_key = None
+        self._dev_small = None
+
+    # -- protocol -------------------------------------------------------------
+    def kernels(self) -> Tuple[str, ...]:
+        return tuple(k.kernel_id for k in self._kernel_bank)
+
+    @property
+    def kernel_bank(self) -> Tuple[Kernel, ...]:
+        return self._kernel_bank
+
+    @property
+    def layer2_state(self) -> Layer2State:
+        return self._layer2
+
+    def _validate(self, cube, validity_mask):
+        if cube.dim() != 4:
+            raise ValueError(f"cube.dim()={cube.dim()}, expected 4 [T_det, N_fdm, H, W]; got shape "
+                             f"{tuple(cube.shape)}")
+        if cube.shape[2] != cube.shape[3]:
+            raise ValueError(f"cube spatial axes must be square (H == W); got H={cube.shape[2]}, W={cube.shape[3]}")
+        if validity_mask.dim() != 2:
+            raise ValueError(f"validity_mask.dim()={validity_mask.dim()}, expected 2 [T_det, N_fdm]; got shape "
+                             f"{tuple(validity_mask.shape)}")
+        if validity_mask.shape[0] != cube.shape[0]:
+            raise ValueError(f"validity_mask.shape[0]={validity_mask.shape[0]} != cube.shape[0]={cube.shape[0]} (T_det)")
+        if validity_mask.shape[1] != cube.shape[1]:
+            raise ValueError(f"validity_mask.shape[1]={validity_mask.shape[1]} != cube.shape[1]={cube.shape[1]} (N_fdm)")
+        if validity_mask.dtype != torch.bool:
+            raise TypeError(f"validity_mask.dtype={validity_mask.dtype}, expected torch.bool")
+        if cube.dtype not in (torch.float16, torch.float32):
+            raise TypeError(f"cube.dtype={cube.dtype}, expected float16 or float32")
+
+    def _sample_cells(self, n_out: int, F: int, HW: int) -> torch.Tensor:
+        """Fixed seeded sample of interior cells ((t, f, pixel) flat indices),
+        restricted to active pixels when ``spatial_active`` is given."""
+        key = (n_out, F, HW)
+        if self._samples_key != key:
+            total = n_out * F * HW
+            rng = np.random.default_rng(0x5A3C)
+            if self._spatial_active is not None and self._spatial_active.size == HW:
+                pix = np.flatnonzero(self._spatial_active)
+                m = min(self._n_samples, n_out * F * pix.size)
+                t = rng.integers(0, n_out, m)
+                f = rng.integers(0, F, m)
+                p = pix[rng.integers(0, pix.size, m)]
+                cells = (t * F + f) * HW + p
+            elif total <= self._n_samples:
+                cells = np.arange(total)
+            else:
+                cells = rng.integers(0, total, self._n_samples)
+            self._samples = torch.from_numpy(cells.astype(np.int64)).to(self._device)
+            self._samples_out = torch.empty((len(self._widths), cells.size), dtype=torch.float32,
+                                            device=self._device)
+            self._samples_key = key
+        return self._samples
+
+    def forward(self, cube: torch.Tensor, validity_mask: torch.Tensor, sigma_layer1: torch.Tensor, *,
+                dm_idx_canonical_lo: Optional[int] = None, dm_idx_canonical_hi: Optional[int] = None,
+                n_kernel_max_t: Optional[int] = None, event_specnum: int = 0,
+                fine_to_coarse: Optional[torch.Tensor] = None,
+                fine_dm_pc_cm3: Optional[torch.Tensor] = None) -> List[Candidate]:
+        self._validate(cube, validity_mask)
+        _lib.require_gpu(cube.contiguous())
+        T, F, H, W = (int(x) for x in cube.shape)
+        n_cell = F * H * W
+        w_max = int(self._widths.max())
+        h = w_max // 2
+        t_edge = max(0, int(self._n_kernel_max_t if n_kernel_max_t is None else n_kernel_max_t) // 2)
+        t_base = t_edge if (t_edge > 0 and 2 * t_edge < T) else 0
+        n_out = T - 2 * t_base
+        x = cube.contiguous()
+        if self._cs is None or tuple(self._cs.shape) != (T + 2 * h + 1, n_cell):
+            self._cs = torch.empty((T + 2 * h + 1, n_cell), dtype=torch.float32, device=x.device)
+        if self._ws is None:
+            self._ws = torch.empty(256 + 16 * self._queue_records, dtype=torch.uint8, device=x.device)
+            self._top = torch.empty((4096, 4), dtype=torch.int32, device=x.device)
+            self._counts = torch.zeros(2, dtype=torch.int32, device=x.device)
+        s_k = self._layer2.s_k                           # earlier cubes' sigma_k
+        inv = (1.0 / np.maximum(s_k, 1e-30)).astype(np.float32)
+        P = _lib._P
+        stream = _lib.stream()
+        _lib.call("dsx_cube_cumsum", _lib.ptr(x), _lib.DSX_F16 if x.dtype == torch.float16 else _lib.DSX_F32,
+                  _lib.ptr(self._cs), T, n_cell, h, stream)
+        _lib.call("dsx_cube_candidates", _lib.ptr(self._cs), T, n_cell, h, self._widths.ctypes.data_as(P),
+                  inv.ctypes.data_as(P), len(self._widths), float(self._threshold), t_base, n_out,
+                  min(self._n_top, 4096), _lib.ptr(self._top), _lib.ptr(self._counts[:1]),
+                  _lib.ptr(self._counts[1:]), _lib.ptr(self._ws), int(self._ws.numel()), stream)
+        cells = self._sample_cells(n_out, F, H * W)
+        _lib.call("dsx_cube_box_samples", _lib.ptr(self._cs), T, n_cell, h, self._widths.ctypes.data_as(P),
+                  len(self._widths), t_base, n_out, _lib.ptr(cells), int(cells.numel()),
+                  _lib.ptr(self._samples_out), stream)
+        counts = self._counts.cpu().numpy()
+        m = int(counts[0])
+        self.last_n_above = int(counts[1])
+        self.last_overflow = self.last_n_above > self._queue_records
+        rec = self._top[:m].cpu().numpy().copy().view(CUBE_REC_DTYPE).reshape(-1)
+        samples = self._samples_out.cpu().numpy()
+        # Layer-2 update from this cube (used from the next cube on)
+        if self._valid_min_fraction >= 1.0:
+            cube_valid = bool(torch.all(validity_mask).item())
+        else:
+            cube_valid = float(validity_mask.float().mean().item()) >= self._valid_min_fraction
+        warming = self._layer2.is_warming_up
+        sig_this = np.array([sigma_clipped_std_host(samples[k]) for k in range(len(self._widths))])
+        self._layer2.update_and_query(per_kernel_sigma=sig_this, valid=cube_valid)
+        # merge-radius decode of the strongest cells (already in (snr, flat index) order)
+        rt, rf, rl = self._radii
+        f2c = None if fine_to_coarse is None else fine_to_coarse.detach().cpu().numpy()
+        fdm = None if fine_dm_pc_cm3 is None else fine_dm_pc_cm3.detach().cpu().numpy()
+        acc, out = [], []
+        flags = int(CandidateFlags.NOISE_WARMUP) if warming else int(CandidateFlags.NONE)
+        for r in rec:
+            v = float(r["snr"])
+            if v <= self._threshold:
+                break
+            t = int(r["t"])
+            f, rem = divmod(int(r["cell"]), H * W)
+            l_, m_ = divmod(rem, W)
+            if any(abs(t - a) <= rt and abs(f - b) <= rf and abs(l_ - c) <= rl and abs(m_ - d) <= rl
+                   for a, b, c, d in acc):
+                continue
+            acc.append((t, f, l_, m_))
+            k = int(r["k"])
+            w = int(self._widths[k])
+            out.append(Candidate(l=float(l_), m=float(m_),
+                                 dm_fine=float(fdm[f]) if fdm is not None else float(f),
+                                 dm_idx=int(f2c[f]) if f2c is not None else int(f),
+                                 event_specnum=int(event_specnum) + t_base + t, width_samples=w,
+                                 kernel_id=self._kernel_bank[k].kernel_id, snr=v,
+                                 detector_version=self._detector_version, flags=flags,
+                                 search_node_id=self._search_node_id, gpu_half=self._gpu_half,
+                                 time_bin=t_base + t))
+        if dm_idx_canonical_lo is not None and dm_idx_canonical_hi is not None:
+            out = [c for c in out if dm_idx_canonical_lo <= c.dm_idx < dm_idx_canonical_hi]
+        return out
+
+    __call__ = forward

@@ -104,6 +104,31 @@ int dsx_image_subbands(const int32_t *vis, const int32_t *tap_src, const int32_t
                        int per_sub, int nbase, int n_grid, int n_rows, int npad,
                        float scale, int img_dtype, dsx_stream_t stream);
 
+/* Fused dequant + subband shift-and-add + C2R imager for a search cube.
+ * For every trial f < n_fdm and cube time t in [t_lo, t_det):
+ *   G[x][y]    = sum_g c_g(streams[g][t - shifts[f][g]][.][x][y])  (chgroups whose
+ *                row t - shifts[f][g] lies in [0, t_stream); others add nothing)
+ *   cube[t][f] = mask * fftshift(Re(ifft2(prescale * G)))           (numpy 1/N^2)
+ * streams is (n_chg, t_stream, 2, N, N) int8 (real plane, then imaginary);
+ * shifts is the device (n_fdm, n_chg) int32 table.  c_g(z) = z summed exactly
+ * in int32 when cal_scale, cal_off_re and cal_off_im are all NULL, else
+ * scale * z + offset in float32 with per-chgroup (cal_2d = 0: [n_chg]) or
+ * per-(chgroup, stream row) (cal_2d = 1: [n_chg][t_stream]) scale and
+ * offsets (NULL: 1 / 0).  mask is [N][N] float32 (or [n_fdm][N][N] with
+ * mask_per_fdm), NULL for none.  cube is (t_det, n_fdm, N, N) of cube_dtype
+ * DSX_F32 or DSX_F16; rows t < t_lo are not written.  The uv grid is never
+ * written to HBM: each work-group folds the Hermitian half plane straight
+ * from the int8 rows into LDS.  An fp16 cube needs acc, a float32 buffer of
+ * the cube's shape for the partial images of the uv-row passes (N = 256:
+ * four passes).
+ * Replaces GpuImager.process_cube (dense path),
+ * src/dsart/image/imager_gpu.py:217-657, with its combine
+ * src/dsart/image/fused_combine_cuda.py:1104-1359. */
+int dsx_image_cube(const int8_t *streams, const int32_t *shifts, int n_chg, int t_stream, int n_fdm,
+                   int t_det, int t_lo, int n_grid, float prescale, const float *cal_scale,
+                   const float *cal_off_re, const float *cal_off_im, int cal_2d, const float *mask,
+                   int mask_per_fdm, void *cube, int cube_dtype, float *acc, dsx_stream_t stream);
+
 /* Dense-grid imager: img[b] = fftshift(Re(ifft2(prescale * grid[b]))) for
  * n_images complex64 grids (n_images, N, N) -> float32 (n_images, N, N),
  * numpy's 1/N^2 normalisation, no mask.
@@ -211,6 +236,46 @@ int dsx_dedisp_prime(const void *ring, int ring_dtype, int n_sub, int n_pix, int
                      int64_t T0, const int32_t *shift, int n_dm, const int32_t *valid_from,
                      const double *n_prev, int64_t c_lo, int n_centres, double *mean,
                      dsx_stream_t stream);
+
+/* ------------------------------------------------- detector over a cube
+ * Time prefix of a search cube x (n_t, n_cell) (n_cell = F * H * W cells,
+ * cube_dtype DSX_F16 / DSX_F32) into cs (n_t + 2h + 1, n_cell) float32:
+ * cs[j][c] = sum_{t < j - h} x[t][c] (zeros before the cube, the total after
+ * it), h = w_max / 2 -- the padded cumulative sum a boxcar bank reads.
+ * Replaces precompute_padded_cumsum, src/dsart/detector/forward.py:250-336. */
+int dsx_cube_cumsum(const void *cube, int cube_dtype, float *cs, int n_t, int64_t n_cell, int h,
+                    dsx_stream_t stream);
+
+/* Candidates of one cube from its prefix cs (dsx_cube_cumsum): for every
+ * centre t in [t_base, t_base + n_out) and cell c, the centred boxcars
+ * B_k = cs[t - w_k/2 + w_k + h][c] - cs[t - w_k/2 + h][c] (widths
+ * host_widths[k], k < n_k <= 16, w_k/2 <= h), snr_k = B_k * host_sigma_inv[k]
+ * (NULL: 1), the maximum over k (strict >, ties -> lowest k).  Cells whose
+ * maximum beats threshold are queued in the workspace (256-byte counter head,
+ * then 16-byte records; workspace_bytes sets the capacity); the
+ * min(n_top, queued) strongest, ordered by (snr descending, flat index
+ * t * n_cell + c ascending), are written to out as 16-byte records
+ * {int32 t - t_base, int32 c, int32 k, float snr}, with their count in
+ * *n_out_dev and the number of cells above threshold (kept or not) in
+ * *n_above.  n_top <= 4096; n_out * n_cell < 2^32.  A count above the queue
+ * capacity means the queue overflowed and the selection covers only the
+ * queued cells.  Nothing is allocated; the counter is cleared by a kernel.
+ * Replaces multi_boxcar_argmax_triton + decode_topk_argmax_lowmem's top-k,
+ * src/dsart/detector/triton_boxcar.py:193-403,
+ * src/dsart/detector/decoder.py:388-533 (the merge-radius suppression
+ * follows on the host over the n_top records). */
+int dsx_cube_candidates(const float *cs, int n_t, int64_t n_cell, int h, const int32_t *host_widths,
+                        const float *host_sigma_inv, int n_k, float threshold, int t_base, int n_out,
+                        int n_top, void *out, int32_t *n_out_dev, int32_t *n_above, void *workspace,
+                        int64_t workspace_bytes, dsx_stream_t stream);
+
+/* Raw boxcar sums of every width at n_s given cells (device int64 flat
+ * indices (t - t_base) * n_cell + c of the centres, t in [t_base, t_base +
+ * n_out)): out[k][i] (NaN for an index outside the centres).  The sample the
+ * Layer-2 noise estimator clips, src/dsart/noise_norm/layer2.py:60-110. */
+int dsx_cube_box_samples(const float *cs, int n_t, int64_t n_cell, int h, const int32_t *host_widths, int n_k,
+                         int t_base, int n_out, const int64_t *cells, int n_s, float *out,
+                         dsx_stream_t stream);
 
 /* Shift-and-add combine of n_streams float32 streams (n_streams, t_stream,
  * n_inner): out[t][k] = sum_g streams[g][t0 + t - shift[g]][k] for
