@@ -316,3 +316,60 @@ extern "C" int dsx_cube_box_samples(const float *cs, int n_t, int64_t n_cell, in
                                                                                n_s, P, out);
     return check_launch("dsx_cube_box_samples");
 }
+
+// ---------------------------------------------------------------------------
+// No-allocation variant: the counter and the queue live in a file-scope device
+// array (each GPU holds its own copy until the process ends).  Host code uses
+// the per-device address from hipGetSymbolAddress, never the array's name.
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+namespace dsx {
+
+constexpr int64_t CD_STATIC_RECORDS = 1 << 20;                    // 16 MiB of queue per GPU
+constexpr int64_t CD_STATIC_BYTES = CD_SCRATCH_HEAD + CD_STATIC_RECORDS * (int64_t)sizeof(CubeRec);
+constexpr int CD_MAX_DEVICES = 64;
+
+static __device__ __align__(256) unsigned char scratch[CD_STATIC_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[CD_MAX_DEVICES];
+
+// Address of this device's scratch, looked up once per device with the device current.
+static int device_scratch(char **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_error(DSX_ELAUNCH, "dsx_cube_candidates_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= CD_MAX_DEVICES)
+        return set_error(DSX_ERANGE, "dsx_cube_candidates_noalloc: device %d >= %d", dev, CD_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return set_error(DSX_ELAUNCH, "dsx_cube_candidates_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<char *>(g_scratch_ptr[dev]);
+    return DSX_OK;
+}
+
+}  // namespace dsx
+
+extern "C" int dsx_cube_candidates_noalloc(const float *cs, int n_t, int64_t n_cell, int h,
+                                           const int32_t *host_widths, const float *host_sigma_inv, int n_k,
+                                           float threshold, int t_base, int n_out, int n_top, void *out,
+                                           int32_t *n_out_dev, int32_t *n_above, dsx_stream_t stream) {
+    DSX_REQUIRE(cs && host_widths && out && n_out_dev && n_above, "dsx_cube_candidates_noalloc: null pointer");
+    DSX_REQUIRE(n_t > 0 && n_cell > 0 && h >= 0, "dsx_cube_candidates_noalloc: bad shape");
+    DSX_REQUIRE(n_top >= 1 && n_top <= CD_MAX_TOP, "dsx_cube_candidates_noalloc: n_top=%d beyond the maximum %d",
+                n_top, CD_MAX_TOP);
+    char *scr = nullptr;
+    const int rc = device_scratch(&scr);
+    if (rc != DSX_OK) return rc;
+    return cube_candidates_impl(cs, n_t, n_cell, h, host_widths, host_sigma_inv, n_k, threshold, t_base, n_out,
+                                n_top, out, n_out_dev, n_above, scr, CD_STATIC_BYTES, as_stream(stream),
+                                "dsx_cube_candidates_noalloc");
+}
+
+extern "C" int64_t dsx_cube_candidates_noalloc_capacity(void) { return CD_STATIC_RECORDS; }

@@ -42,6 +42,8 @@ PROTOTYPES = {
     "dsx_cube_cumsum": [_P, _I, _P, _I, _I64, _I, _P],
     "dsx_cube_candidates": [_P, _I, _I64, _I, _P, _P, _I, _F, _I, _I, _I, _P, _P, _P, _P, _I64, _P],
     "dsx_cube_box_samples": [_P, _I, _I64, _I, _P, _I, _I, _I, _P, _I, _P, _P],
+    "dsx_cube_candidates_noalloc": [_P, _I, _I64, _I, _P, _P, _I, _F, _I, _I, _I, _P, _P, _P, _P],
+    "dsx_cube_candidates_noalloc_capacity": [],
     "dsx_synth_block": [_P, _U64, _I64, _I, _I, _I, _P, _I, _P, _P, _P, _P],
     "dsx_boxcar": [_P, _P, _I, _I, _I, _P],
     "dsx_multi_boxcar_argmax": [_P, _I, _P, _P, _P, _I, _P, _I, _P, _I, _I, _I, _I, _P],
@@ -62,7 +64,8 @@ def lib() -> ctypes.CDLL:
         for name, argtypes in PROTOTYPES.items():
             fn = getattr(handle, name)
             fn.argtypes = argtypes
-            fn.restype = ctypes.c_char_p if name == "dsx_last_error" else ctypes.c_int
+            fn.restype = (ctypes.c_char_p if name == "dsx_last_error" else
+                          ctypes.c_int64 if name.endswith("_capacity") else ctypes.c_int)
         _lib = handle
     return _lib
 

@@ -269,6 +269,25 @@ int dsx_cube_candidates(const float *cs, int n_t, int64_t n_cell, int h, const i
                         int n_top, void *out, int32_t *n_out_dev, int32_t *n_above, void *workspace,
                         int64_t workspace_bytes, dsx_stream_t stream);
 
+/* dsx_cube_candidates with no allocation and no workspace argument, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It has no
+ * counterpart in the reference.  The counter and the queue live in a static
+ * device array of this library, 16 MiB per GPU held until the process ends
+ * (capacity dsx_cube_candidates_noalloc_capacity() = 2^20 queued cells;
+ * *n_above beyond it reports an overflowed queue, as with a workspace of
+ * that size); n_top beyond 4096 is rejected.  The counter is cleared by a
+ * kernel at every call.  Every call on one GPU shares that GPU's array, so
+ * calls that overlap on one GPU (concurrent streams or threads) race; the
+ * caller serialises them.  Results are those of dsx_cube_candidates. */
+int dsx_cube_candidates_noalloc(const float *cs, int n_t, int64_t n_cell, int h, const int32_t *host_widths,
+                                const float *host_sigma_inv, int n_k, float threshold, int t_base, int n_out,
+                                int n_top, void *out, int32_t *n_out_dev, int32_t *n_above,
+                                dsx_stream_t stream);
+
+/* Queue capacity (records) of dsx_cube_candidates_noalloc. */
+int64_t dsx_cube_candidates_noalloc_capacity(void);
+
 /* Raw boxcar sums of every width at n_s given cells (device int64 flat
  * indices (t - t_base) * n_cell + c of the centres, t in [t_base, t_base +
  * n_out)): out[k][i] (NaN for an index outside the centres).  The sample the

@@ -18,7 +18,7 @@ HEADER = os.path.join(ROOT, "include", "dsx.h")
 def _declared():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(?:int|const char \*)\s*(dsx_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(?:int|int64_t|const char \*)\s*(dsx_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_built():
