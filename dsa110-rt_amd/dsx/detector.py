@@ -21,6 +21,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .merger import MergerConfig, merge_across_kernels_c1
 
 DETECTOR_VERSION = "dsx.v1"
 
@@ -88,13 +89,14 @@ def device_records(rec_dev: np.ndarray, c_lo: int) -> np.ndarray:
 
 def decode_chunk_maxima(rec: np.ndarray, *, widths: Sequence[int], n_grid: int,
                         dm_trials: np.ndarray, threshold: float, n_top: int, radius_t: int,
-                        radius_dm: int, radius_lm: int, t_int: int,
+                        radius_dm: int, radius_lm: int, specnum_start: int = 0,
                         search_node_id: int = 0, gpu_half: int = 0) -> List[Candidate]:
     """Top-k over above-threshold chunk maxima, ordered by (-snr, t, dm, pixel)
     (the flat-index order of decoder.py:441-443 on a [T, F, H, W] cube), then
     greedy suppression of anything within ``|dt| <= radius_t``,
     ``|d dm| <= radius_dm``, ``|d row|, |d col| <= radius_lm`` of an accepted
-    candidate (decoder.py:471-499)."""
+    candidate (decoder.py:471-499).  ``event_specnum`` counts search samples
+    (dedispersed steps) from ``specnum_start``, as ``decoder.py:216`` does."""
     if rec.size == 0:
         return []
     rec = rec[rec["snr"] > threshold]
@@ -113,7 +115,7 @@ def decode_chunk_maxima(rec: np.ndarray, *, widths: Sequence[int], n_grid: int,
         w = int(widths[int(rec["k"][o])])
         out.append(Candidate(
             l=float(r), m=float(c), dm_fine=float(dm_trials[d]), dm_idx=d,
-            event_specnum=t * (t_int // 2), width_samples=w, kernel_id=f"unit:d1:b{w}",
+            event_specnum=int(specnum_start) + t, width_samples=w, kernel_id=f"unit:d1:b{w}",
             snr=float(rec["snr"][o]), detector_version=DETECTOR_VERSION,
             flags=int(CandidateFlags.NONE), search_node_id=search_node_id, gpu_half=gpu_half,
             time_bin=t))
@@ -341,8 +343,13 @@ class DeterministicDetector:
                  layer2_sigma_max_ratio: float = 0.0, layer2_clamp_escape_cubes: int = 0,
                  layer2_valid_min_fraction: float = 1.0, layer2_sigma_max_samples: int = 1 << 16,
                  streaming_decoder_n_top: int = 64, spatial_active: Optional[np.ndarray] = None,
-                 queue_records: int = 1 << 20):
+                 queue_records: int = 1 << 20, merger_config: Optional[MergerConfig] = None,
+                 c1_snr_min: Optional[float] = None):
         ws = [int(w) for w in time_widths]
+        if c1_snr_min is not None:                       # the C1 floor is the decode threshold
+            threshold_sigma = float(c1_snr_min)
+        self._merger_config = merger_config
+        self._c1_snr_min = None if c1_snr_min is None else float(c1_snr_min)
         if not ws or any(w < 1 for w in ws) or len(ws) > 16:
             raise ValueError(f"time_widths={tuple(ws)}: 1-16 positive widths")
         self._kernel_bank = tuple(Kernel(kernel_id=f"unit:d1:b{w}", k_time_width=w) for w in ws)
@@ -502,8 +509,16 @@ class DeterministicDetector:
                                  detector_version=self._detector_version, flags=flags,
                                  search_node_id=self._search_node_id, gpu_half=self._gpu_half,
                                  time_bin=t_base + t))
+        if self._merger_config is not None:              # C1 geometry over the decoded survivors
+            out = merge_across_kernels_c1(out, self._merger_config)
         if dm_idx_canonical_lo is not None and dm_idx_canonical_hi is not None:
             out = [c for c in out if dm_idx_canonical_lo <= c.dm_idx < dm_idx_canonical_hi]
+        if self._c1_snr_min is not None:
+            out = [c for c in out if float(c.snr) >= self._c1_snr_min]
         return out
+
+    @property
+    def merger_config(self) -> Optional[MergerConfig]:
+        return self._merger_config
 
     __call__ = forward

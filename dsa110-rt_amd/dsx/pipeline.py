@@ -445,7 +445,7 @@ class SearchNode:
         rec = device_records(rec_dev, c_lo)
         cands = decode_chunk_maxima(rec, widths=cfg.widths, n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
                                     threshold=thr, n_top=cfg.n_top, radius_t=cfg.radius_t,
-                                    radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm, t_int=cfg.t_int)
+                                    radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm)
         return BlockResult(candidates=cands, n_raw=n, overflow=n > self.cand_cap, T0=pend.T0,
                            n_steps=pend.n_steps, records=rec, threshold_used=thr)
 
@@ -631,6 +631,6 @@ class MultiRangeNode:
         rec = np.concatenate(recs) if recs else np.zeros(0, REC_DTYPE)
         cands = decode_chunk_maxima(rec, widths=self.widths_all, n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
                                     threshold=cfg.threshold, n_top=cfg.n_top, radius_t=cfg.radius_t,
-                                    radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm, t_int=cfg.t_int)
+                                    radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm)
         return BlockResult(candidates=cands, n_raw=int(n_raw), overflow=overflow, T0=int(T0), n_steps=n_steps,
                            records=rec, threshold_used=max(thr_used), range_thresholds=thr_used)

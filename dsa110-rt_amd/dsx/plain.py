@@ -196,7 +196,7 @@ class PlainNode:
         rec = np.concatenate(recs) if recs else np.zeros(0, REC_DTYPE)
         cands = decode_chunk_maxima(rec, widths=widths, n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
                                     threshold=cfg.threshold, n_top=cfg.n_top, radius_t=cfg.radius_t,
-                                    radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm, t_int=cfg.t_int)
+                                    radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm)
         self.T0 = T0 + n_steps
         return BlockResult(candidates=cands, n_raw=int(rec.size), overflow=False, T0=T0, n_steps=n_steps,
                            records=rec)

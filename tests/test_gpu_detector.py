@@ -83,3 +83,34 @@ def test_detector_validation_and_invalid_cube():
     bad[3, 1] = False
     det.forward(torch.randn((16, 2, 8, 8), device="cuda"), bad, None)
     assert np.array_equal(det.layer2_state.s_k, s0) and det.layer2_state.cube_count == 0
+
+
+def test_detector_c1_merger_and_wire():
+    """With a MergerConfig the detector's survivors are the oracle decode's
+    candidates after the pairwise C1 merge, and they encode to a batch
+    that parses back to the same rows."""
+    from dsx import wire
+    from dsx.merger import MergerConfig
+    from oracle import merger as om
+    rng = np.random.default_rng(5)
+    T, F, N = 96, 6, 16
+    mc = MergerConfig(lm_max_cells=1, dm_max_trials=1, sample_period_specnum=1)
+    kw = dict(time_widths=WIDTHS, threshold_sigma=4.5, layer2_n_burnin=2, streaming_decoder_n_top=256,
+              layer2_sigma_max_samples=4096)
+    det = DeterministicDetector(merger_config=mc, **kw)
+    ref = DeterministicDetector(**kw)
+    valid = torch.ones((T, F), dtype=torch.bool, device="cuda")
+    for blk in range(2):
+        xt = torch.from_numpy(_cube(rng, T, F, N, blk)).cuda()
+        got = det.forward(xt, valid, None, event_specnum=1000 * blk)
+        base = ref.forward(xt, valid, None, event_specnum=1000 * blk)
+        want = om.merge_c1(base, lm_max=1, dm_max=1, t_frac=1.0, sps=1.0)
+        assert [c.key for c in got] == [c.key for c in want] and len(got) <= len(base)
+        geom = wire.CubeGeometry(cube_id=blk, specnum_start=1000 * blk, sample_period_specnum=1, t_det=T, n_grid=N,
+                                 n_fdm_in_cube=F, sample_period_us=1048.576, cell_l_rad=1e-3, cell_m_rad=1e-3,
+                                 l0_rad=0.0, m0_rad=0.0, fine_dm_pc_cc=np.arange(F, dtype=np.float64),
+                                 mjd_start=60000.0)
+        blob, surv = wire.cube_batch(got, geom, merger=None)
+        rows = wire.parse_c1_batch(blob.decode().rstrip("\n").split("\n")).candidates
+        assert [(r.l_pix, r.m_pix, r.fine_dm_idx, r.event_specnum, r.width_samples) for r in rows] == \
+            [(int(c.l), int(c.m), c.dm_idx, c.event_specnum, c.width_samples) for c in surv]

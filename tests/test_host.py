@@ -130,8 +130,7 @@ def test_decode_chunk_maxima_matches_oracle_decode():
     rec["snr"] = rng.uniform(8, 30, n).astype(np.float32)
     cfg = dcfg.small_node_config()
     got = decode_chunk_maxima(rec, widths=cfg.widths, n_grid=64, dm_trials=cfg.dm_trials,
-                              threshold=8.0, n_top=100, radius_t=128, radius_dm=5, radius_lm=3,
-                              t_int=8)
+                              threshold=8.0, n_top=100, radius_t=128, radius_dm=5, radius_lm=3)
     # same records through the oracle decoder (one chunk per record)
     bs = np.full((64, n, 64 * 64), -np.inf)
     bt = np.zeros(bs.shape, dtype=np.int32)
