@@ -898,3 +898,26 @@ extern "C" int dsx_dedisp_prime(const void *ring, int ring_dtype, int n_sub, int
             n_prev, c_lo, n_centres, mean);
     return check_launch("dsx_dedisp_prime");
 }
+
+// Exponential forgetting of the running statistics: after a block's merge,
+// the state's weight n_prev[d] and M2 m2[p][d] are multiplied by `decay`
+// (the mean is a ratio and stays).  Each earlier block then weighs
+// decay^age in later merges and the sigma sqrt(m2 / n) is unchanged by the
+// scaling itself.
+__global__ void __launch_bounds__(256) stats_decay_kernel(double *__restrict__ m2, int64_t n_m2,
+                                                          double *__restrict__ n_prev, int n_dm, double decay) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_m2; i += stride) m2[i] *= decay;
+    if (blockIdx.x == 0)
+        for (int d = threadIdx.x; d < n_dm; d += blockDim.x) n_prev[d] *= decay;
+}
+
+extern "C" int dsx_stats_decay(double *m2, int64_t n_m2, double *n_prev, int n_dm, double decay,
+                               dsx_stream_t stream) {
+    DSX_REQUIRE(m2 && n_prev, "dsx_stats_decay: null pointer");
+    DSX_REQUIRE(n_m2 > 0 && n_dm > 0, "dsx_stats_decay: sizes");
+    DSX_REQUIRE(decay > 0.0 && decay <= 1.0, "dsx_stats_decay: decay=%g outside (0, 1]", decay);
+    const int64_t blocks = std::min<int64_t>((n_m2 + 255) / 256, 4096);
+    stats_decay_kernel<<<(unsigned)blocks, 256, 0, as_stream(stream)>>>(m2, n_m2, n_prev, n_dm, decay);
+    return check_launch("dsx_stats_decay");
+}

@@ -7,6 +7,7 @@ CPU-sized case and the 96-antenna node.
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field, replace
 from typing import Optional, Tuple
 
@@ -50,6 +51,10 @@ class NodeConfig:
     widths: Tuple[int, ...] = (1, 2, 4, 8)
     threshold: float = 8.0
     min_stat_count: int = 512              # complete samples per DM before it is scored
+    #: e-folding time of the running statistics (None: cumulative for ever);
+    #: after each block the state's weight and M2 are scaled by
+    #: exp(-block duration / stats_tau_s)
+    stats_tau_s: Optional[float] = None
     n_top: int = 256
     radius_t: int = 128
     radius_dm: int = 5
@@ -155,6 +160,13 @@ class NodeConfig:
     def w_max(self) -> int:
         return int(max(self.widths))
 
+    def stats_decay(self, n_steps: int) -> float:
+        """Weight kept by the running statistics after a block of ``n_steps``
+        steps: ``exp(-n_steps * t_step / stats_tau_s)``, 1 without forgetting."""
+        if self.stats_tau_s is None:
+            return 1.0
+        return math.exp(-n_steps * self.t_step_us * 1e-6 / float(self.stats_tau_s))
+
     def validate(self, detect: bool = True) -> None:
         """Check shapes; ``detect`` adds the block/width rules of the detector."""
         if self.n_ant < 2:
@@ -182,6 +194,14 @@ class NodeConfig:
             raise ValueError("widths must be 1, 2, 4, ... without gaps")
         if self.w_max > 2 * DETECT_DELAY_STEPS:
             raise ValueError("widest boxcar exceeds twice the detection delay")
+        if self.stats_tau_s is not None:
+            if not self.stats_tau_s > 0:
+                raise ValueError(f"stats_tau_s={self.stats_tau_s} must be > 0")
+            lam = self.stats_decay(self.n_steps)
+            n_sat = self.n_steps * lam / (1.0 - lam)        # weight the state saturates at
+            if n_sat < self.min_stat_count:
+                raise ValueError(f"stats_tau_s={self.stats_tau_s}: the decayed sample weight saturates at "
+                                 f"{n_sat:.0f} < min_stat_count={self.min_stat_count}; no DM would be scored")
         for lo, hi, n, f, _ in self.dm_ranges:
             if int(f) < 1 or self.n_steps % (DETECT_CHUNK_STEPS * int(f)) != 0:
                 raise ValueError(f"DM range {lo}-{hi}: downsampling {f} must leave a multiple of "

@@ -294,6 +294,7 @@ class SearchNode:
         si, so = self.stat_cur, (self.stat_cur + 1) % self.n_stat
         seq = self.block_seq
         cfg = self.cfg
+        lam = cfg.stats_decay(n_steps)
         with torch.cuda.stream(s):
             self._blk_pinned[slot].copy_(torch.from_numpy(counts))
             self.blk_count.copy_(self._blk_pinned[slot], non_blocking=True)
@@ -312,6 +313,9 @@ class SearchNode:
                                 cand, n_cand, T0, n_steps, s, float(self.cfg.threshold), self.cur)
             self._mark("dedisp", 1)
             torch.add(self.n_prev_bufs[si], self.blk_count, out=self.n_prev_bufs[so])
+            if lam < 1.0:
+                _lib.call("dsx_stats_decay", _lib.ptr(self.m2_bufs[so]), self.n_pix * cfg.n_dm,
+                          _lib.ptr(self.n_prev_bufs[so]), cfg.n_dm, lam, _lib.stream(s))
             self._cnt_pinned[slot].copy_(n_cand, non_blocking=True)
             self._rec_pinned[slot].copy_(cand[:self._rec_prefix], non_blocking=True)
             done = torch.cuda.Event()
@@ -319,6 +323,7 @@ class SearchNode:
         self.stat_writer[so] = seq
         self.stat_cur = so
         self.n_prev_host += counts
+        self.n_prev_host *= lam
         self.T0 = T0 + n_steps
         self.block_seq = seq + 1
         return PendingBlock(slot=slot, T0=T0, n_steps=n_steps, done=done, stream=s, seq=seq, stat_in=si,

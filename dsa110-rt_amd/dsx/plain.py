@@ -193,6 +193,10 @@ class PlainNode:
         self.mean = torch.where(upd, self.mean + delta * (nb / safe_n), self.mean)
         self.m2 = torch.where(upd, self.m2 + m2_b + delta * delta * (n_a * nb / safe_n), self.m2)
         self.n_prev += counts
+        lam = cfg.stats_decay(n_steps)                 # forgetting of the running statistics
+        if lam < 1.0:
+            self.m2 = self.m2 * lam
+            self.n_prev *= lam
         rec = np.concatenate(recs) if recs else np.zeros(0, REC_DTYPE)
         cands = decode_chunk_maxima(rec, widths=widths, n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
                                     threshold=cfg.threshold, n_top=cfg.n_top, radius_t=cfg.radius_t,

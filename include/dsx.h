@@ -237,6 +237,15 @@ int dsx_dedisp_prime(const void *ring, int ring_dtype, int n_sub, int n_pix, int
                      const double *n_prev, int64_t c_lo, int n_centres, double *mean,
                      dsx_stream_t stream);
 
+/* Exponential forgetting of the running statistics, applied after a block's
+ * merge: m2[i] *= decay for i < n_m2 and n_prev[d] *= decay for d < n_dm
+ * (0 < decay <= 1; the mean stays).  Each earlier block then weighs decay^age
+ * in later merges, an EMA of the per-(pixel, DM) moments with
+ * decay = exp(-block duration / tau) -- the forgetting of the reference's
+ * Layer-2 sigma EMA, src/dsart/noise_norm/layer2.py:313-452
+ * (gamma = 1 - exp(-cadence / tau)). */
+int dsx_stats_decay(double *m2, int64_t n_m2, double *n_prev, int n_dm, double decay, dsx_stream_t stream);
+
 /* ------------------------------------------------- detector over a cube
  * Time prefix of a search cube x (n_t, n_cell) (n_cell = F * H * W cells,
  * cube_dtype DSX_F16 / DSX_F32) into cs (n_t + 2h + 1, n_cell) float32:

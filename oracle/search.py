@@ -13,6 +13,7 @@ against ``search_block_np`` (pure numpy) on small inputs.
 """
 from __future__ import annotations
 
+import math
 import ctypes
 import os
 import subprocess
@@ -191,6 +192,20 @@ def merge_stats(n_a, mean_a, m2_a, n_b, sum_b, sumsq_b):
     mean = np.where(nb > 0, mean_a + delta * (nb / safe_n), mean_a)
     m2 = np.where(nb > 0, m2_a + m2_b + delta * delta * (n_a[:, None] * nb / safe_n), m2_a)
     return n_a + n_b, mean, m2
+
+
+def block_decay(n_steps: int, t_int: int, tau_s: float) -> float:
+    """Weight the running statistics keep after a block of ``n_steps`` steps
+    of ``t_int`` native samples: ``exp(-duration / tau_s)``, the forgetting
+    factor of an EMA with time constant ``tau_s`` (the reference's Layer-2
+    ``gamma = 1 - exp(-cadence / tau)``, ``src/dsart/noise_norm/layer2.py:190-230``)."""
+    return math.exp(-n_steps * t_int * NATIVE_US * 1e-6 / float(tau_s))
+
+
+def decay_stats(n, m2, lam):
+    """Forget after a merge: weight and M2 scaled by ``lam`` (the mean is a
+    ratio and stays), so each earlier block weighs ``lam**age``."""
+    return np.asarray(n, dtype=np.float64) * lam, np.asarray(m2, dtype=np.float64) * lam
 
 
 @dataclass

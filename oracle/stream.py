@@ -103,6 +103,10 @@ class OracleStream:
             if self.pix.size == c.n_grid ** 2 else []
         self.st_n, self.st_mean, self.st_m2 = osearch.merge_stats(
             self.st_n, self.st_mean, self.st_m2, cnt, s1, s2)
+        tau = getattr(c, "stats_tau_s", None)
+        if tau is not None:
+            self.st_n, self.st_m2 = osearch.decay_stats(
+                self.st_n, self.st_m2, osearch.block_decay(L, c.t_int, tau))
         return cands
 
     def push_raw(self, raw: np.ndarray, n_samp: int):

@@ -237,3 +237,39 @@ def test_record_times_past_int32():
         assert all(k[1] >= T_start - 64 for k in g)
         assert all(c.time_bin >= T_start - 64 for c in res.candidates)
     assert total > 0
+
+
+def test_stats_decay_vs_oracle():
+    """Running statistics with forgetting (e-folding 0.1 s, about half a
+    block): six blocks whose noise level triples after the third; the GPU's
+    weights, means, M2 and records equal the oracle's, and the state's sigma
+    moves most of the way to the new level within three blocks (a
+    cumulative state would stay near the old one)."""
+    cfg = dcfg.small_node_config().with_(n_sub=8, n_grid=16, dm_max=100.0, n_dm=64, threshold=4.5,
+                                         min_stat_count=128, widths=(1, 2, 4, 8, 16), stats_tau_s=0.1)
+    rng = np.random.default_rng(31)
+    ant_e = rng.uniform(-500, 500, cfg.n_ant)
+    ant_n = rng.uniform(-100, 100, cfg.n_ant)
+    node = SearchNode(cfg, ant_e, ant_n)
+    o = OracleStream(cfg, ant_e, ant_n)
+    L, N = cfg.n_steps, cfg.n_grid
+    lam = cfg.stats_decay(L)
+    assert 0.4 < lam < 0.6
+    sig_hist = []
+    for blk in range(6):
+        img = (rng.standard_normal((cfg.n_sub, L, N * N)) * (1.0 if blk < 3 else 3.0)).astype(np.float32)
+        node.push_images(torch.from_numpy(img.reshape(cfg.n_sub, L, N, N)).cuda(), L)
+        res = node.detect(L)
+        o.push_images(img.astype(np.float64))
+        _compare_records(_gpu_records(res.records), _oracle_records(o, cfg.threshold), cfg.threshold, 1e-4)
+        n_g = node.n_prev.cpu().numpy()
+        np.testing.assert_allclose(n_g, o.st_n, rtol=1e-12)
+        mean = node.mean.cpu().numpy().T
+        m2 = node.m2.cpu().numpy().T
+        np.testing.assert_allclose(mean, o.st_mean, rtol=1e-5, atol=6e-6 * np.sqrt(cfg.n_sub))
+        np.testing.assert_allclose(m2, o.st_m2, rtol=1e-5, atol=1e-5)
+        ok = n_g > 0
+        sig_hist.append(float(np.median(np.sqrt(m2[ok] / n_g[ok, None]))))
+    s_lo = np.sqrt(cfg.n_sub)                  # unit-variance images: sigma of the dedispersed series
+    assert abs(sig_hist[2] / s_lo - 1.0) < 0.1
+    assert sig_hist[5] / s_lo > 2.5           # cumulative weighting would give about sqrt(5) = 2.24
