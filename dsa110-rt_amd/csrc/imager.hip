@@ -82,6 +82,7 @@ struct ImgParams {
     int n_steps, n_sub, per_sub, nbase, N, log2N, n_rows, rc_max, npad;
     float scale;
     int dense;                // 0: taps from int32 visibilities (+ mask); 1: dense grid, no mask;
+                              // 3: taps from float32 visibilities (calibrated streams, + mask);
                               // 2: shift-and-add of cint8 uv streams (cube mode, below)
     // cube mode: item (f, t), t in [t_lo, t_lo + n_steps), f < n_sub:
     //   G[x][y] = sum_g cal_g(streams[g][t - shifts[f][g]][re, im][x][y]) for in-range rows,
@@ -108,8 +109,11 @@ __device__ __forceinline__ float cal_at(const float *a, int cal_2d, int g, int t
 // otherwise radix-2 passes over rows of RS = N entries.
 constexpr int R16_RS = 272;
 
-template <typename TOut, bool R16>
+// MODE (= P.dense) is a compile-time constant so each mode carries only its own registers.
+template <typename TOut, bool R16, int MODE>
 __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
+    constexpr int DENSE = MODE == 3 ? 0 : MODE;     // taps (0), dense grid (1), cube streams (2)
+    constexpr bool VISF = MODE == 3;                // float32 visibilities, float LDS atomics
     extern __shared__ __align__(16) float2 smem[];
     const int N = P.N;
     const int RS = R16 ? R16_RS : N;         // row stride of H
@@ -142,15 +146,15 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
     for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
         const int s = item / P.n_steps;
         const int t = item - s * P.n_steps;
-        const int32_t *vslab = P.dense ? nullptr
+        const int32_t *vslab = DENSE ? nullptr
                                        : P.vis + 2 * ((int64_t)t * n_cout + (int64_t)s * P.per_sub) * P.nbase;
-        const int tap_lo = P.dense ? 0 : P.tap_off[s], tap_hi = P.dense ? 0 : P.tap_off[s + 1];
-        const float2 *gsrc = P.dense == 1 ? P.grid + (int64_t)item * N * N : nullptr;
-        const int64_t o_item = P.dense == 2 ? ((int64_t)(P.t_lo + t) * P.n_sub + s) * N * N
+        const int tap_lo = DENSE ? 0 : P.tap_off[s], tap_hi = DENSE ? 0 : P.tap_off[s + 1];
+        const float2 *gsrc = DENSE == 1 ? P.grid + (int64_t)item * N * N : nullptr;
+        const int64_t o_item = DENSE == 2 ? ((int64_t)(P.t_lo + t) * P.n_sub + s) * N * N
                                             : ((int64_t)s * P.n_steps + t) * N * N;
         TOut *out = img + o_item;
-        float *acc_out = P.acc != nullptr ? P.acc + o_item : nullptr;
-        if (P.dense == 2) {
+        float *acc_out = (DENSE == 2 && P.acc != nullptr) ? P.acc + o_item : nullptr;   // cube mode only
+        if constexpr (DENSE == 2) {
             __syncthreads();                 // the previous item's rows are no longer read
             for (int g = tid; g < P.n_chg; g += IMG_THREADS) {
                 const int tau = P.t_lo + t - P.shifts[(int64_t)s * P.n_chg + g];
@@ -161,7 +165,7 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
             const int r0 = ck * P.rc_max;
             const int rc = min(P.rc_max, P.n_rows - r0);
             __syncthreads();
-            if (P.dense) {
+            if constexpr (DENSE != 0) {
                 // Hermitian fold of (-1)^(a+b) G[a][b]: H[r][w] = (-1)^(r+w) (G[r][w] + conj G[-r][-w]),
                 // written straight into float32 rows in bit-reversed order
                 for (int i = tid; i < rc * N; i += IMG_THREADS) {
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                     const int64_t ca = (int64_t)rg * N + w;
                     const int64_t cb = (int64_t)((N - rg) & (N - 1)) * N + ((N - w) & (N - 1));
                     float2 a, b;
-                    if (P.dense == 2) {
+                    if constexpr (DENSE == 2) {
                         // shift-and-add of the chgroups' cint8 rows at this cell and its mirror
                         const int64_t plane = (int64_t)N * N;
                         if (P.cal_scale == nullptr && P.cal_off_re == nullptr && P.cal_off_im == nullptr) {
@@ -227,6 +231,28 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                     dst[k] = P.tap_dst[ic];
                     src[k] = P.tap_src[ic];
                 }
+                if constexpr (VISF) {
+                    // calibrated visibilities: float32 LDS atomics (sum order not fixed)
+                    float2 v[IMG_TAP_BATCH];
+#pragma unroll
+                    for (int k = 0; k < IMG_TAP_BATCH; ++k)
+                        v[k] = *reinterpret_cast<const float2 *>(vslab + 2 * (int64_t)src[k]);
+#pragma unroll
+                    for (int k = 0; k < IMG_TAP_BATCH; ++k) {
+                        const int d = dst[k];
+                        const int cell = d & TAP_CELL;
+                        const int r = (cell >> P.log2N) - r0;
+                        if (i0 + k * IMG_THREADS < tap_hi && r >= 0 && r < rc) {
+                            const int w = cell & (N - 1);
+                            float2 x = v[k];
+                            if (d & TAP_CONJ) x.y = -x.y;
+                            if (d & TAP_NEG) { x.x = -x.x; x.y = -x.y; }
+                            float *c = reinterpret_cast<float *>(&H[r * RS + w]);
+                            atomicAdd(c, x.x);
+                            atomicAdd(c + 1, x.y);
+                        }
+                    }
+                } else {
                 int2 v[IMG_TAP_BATCH];
 #pragma unroll
                 for (int k = 0; k < IMG_TAP_BATCH; ++k)
@@ -246,6 +272,7 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                         atomicAdd(c + 1, x.y);
                     }
                 }
+                }
             }
             __syncthreads();
             // ---- int32 -> float32 in place, with the bit-reversal permutation
@@ -254,7 +281,13 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
             for (int i = tid; i < rc * N; i += IMG_THREADS) {
                 const int r = i >> P.log2N, j = i & (N - 1);
                 const int jr = (int)(__brev((unsigned)j) >> (32 - P.log2N));
-                if (j < jr) {
+                if constexpr (VISF) {
+                    if (j < jr) {
+                        const float2 a = H[r * N + j], b = H[r * N + jr];
+                        H[r * N + j] = b;
+                        H[r * N + jr] = a;
+                    }
+                } else if (j < jr) {
                     const int2 a = Hi[r * N + j], b = Hi[r * N + jr];
                     H[r * N + j] = make_float2((float)b.x, (float)b.y);
                     H[r * N + jr] = make_float2((float)a.x, (float)a.y);
@@ -270,7 +303,7 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                 // pass 1 (task = row, w2): Z[w2][j1] = e^{2 pi i w2 j1 / 256} sum_w1 H[16 w1 + w2]
                 // e^{2 pi i w1 j1 / 16}, stored at 17 w2 + j1.  The 16 tasks of a row are
                 // 16 lanes of one wave, so a row's reads finish before its writes.
-                const bool h_int = !P.dense;
+                const bool h_int = !DENSE && !VISF;
                 for (int task = tid; task < rc * 16; task += IMG_THREADS) {
                     const int r = task >> 4, w2 = task & 15;
                     float2 x[16];
@@ -388,12 +421,14 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int i = i0 + 4 * lg + v;
-                        const bool inside = P.dense || (i >= P.npad && i < N - P.npad && j >= P.npad &&
+                        const bool inside = DENSE || (i >= P.npad && i < N - P.npad && j >= P.npad &&
                                                         j < N - P.npad && !(i == halfN && j == halfN));
                         const int64_t o = (int64_t)i * N + j;
                         float val = inside ? acc[v] * out_scale : 0.f;
-                        if (P.mask != nullptr) val *= P.mask[(int64_t)s * P.mask_stride + o];
-                        if (acc_out != nullptr) {
+                        if constexpr (DENSE == 2) {
+                            if (P.mask != nullptr) val *= P.mask[(int64_t)s * P.mask_stride + o];
+                        }
+                        if (DENSE == 2 && acc_out != nullptr) {
                             // partial images in float32; the last pass writes the output type
                             if (ck > 0) val += acc_out[o];
                             if (ck + 1 < n_chunks) acc_out[o] = val;
@@ -446,23 +481,33 @@ static size_t img_lds_bytes(int n_grid, int rc_max, bool r16, int n_chg = 0) {
     return ((size_t)n_grid + (r16 ? 256 : 0) + (size_t)rc_max * rs) * sizeof(float2) + (size_t)n_chg * sizeof(int);
 }
 
+template <typename TOut, bool R16, int MODE>
+static int launch_image_m(const ImgParams &P, int grid, size_t lds, hipStream_t s) {
+    (void)hipFuncSetAttribute((const void *)image_kernel<TOut, R16, MODE>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    image_kernel<TOut, R16, MODE><<<grid, IMG_THREADS, lds, s>>>(P);
+    return check_launch("dsx_image_subbands");
+}
+
 template <typename TOut, bool R16>
 static int launch_image(const ImgParams &P, int grid, size_t lds, hipStream_t s) {
-    (void)hipFuncSetAttribute((const void *)image_kernel<TOut, R16>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    image_kernel<TOut, R16><<<grid, IMG_THREADS, lds, s>>>(P);
-    return check_launch("dsx_image_subbands");
+    switch (P.dense) {
+        case 0: return launch_image_m<TOut, R16, 0>(P, grid, lds, s);
+        case 1: return launch_image_m<TOut, R16, 1>(P, grid, lds, s);
+        case 2: return launch_image_m<TOut, R16, 2>(P, grid, lds, s);
+        default: return launch_image_m<TOut, R16, 3>(P, grid, lds, s);
+    }
 }
 
 }  // namespace dsx
 
 using namespace dsx;
 
-extern "C" int dsx_image_subbands(const int32_t *vis, const int32_t *tap_src,
-                                  const int32_t *tap_dst, const int32_t *tap_off, void *img,
-                                  int n_steps, int n_sub, int per_sub, int nbase, int n_grid,
-                                  int n_rows, int npad, float scale, int img_dtype,
-                                  dsx_stream_t stream) {
+static int image_subbands(const void *vis, int vis_f32, const int32_t *tap_src,
+                          const int32_t *tap_dst, const int32_t *tap_off, void *img,
+                          int n_steps, int n_sub, int per_sub, int nbase, int n_grid,
+                          int n_rows, int npad, float scale, int img_dtype,
+                          dsx_stream_t stream) {
     DSX_REQUIRE(vis && tap_src && tap_dst && tap_off && img, "dsx_image_subbands: null pointer");
     DSX_REQUIRE(n_steps > 0 && n_sub > 0 && per_sub > 0 && nbase > 0, "dsx_image_subbands: bad shape");
     DSX_REQUIRE(n_grid >= 16 && n_grid <= 1024 && (n_grid & (n_grid - 1)) == 0,
@@ -481,13 +526,31 @@ extern "C" int dsx_image_subbands(const int32_t *vis, const int32_t *tap_src,
                          "dsx_image_subbands: %d uv rows need %d passes; multi-pass imaging "
                          "accumulates in place and needs img_dtype DSX_F32", n_rows, n_chunks);
     hipStream_t s = as_stream(stream);
-    ImgParams P{vis, tap_src, tap_dst, tap_off, nullptr, img, n_steps, n_sub, per_sub, nbase, n_grid,
-                log2N, n_rows, rc_max, npad, scale, 0};
+    ImgParams P{(const int32_t *)vis, tap_src, tap_dst, tap_off, nullptr, img, n_steps, n_sub, per_sub,
+                nbase, n_grid, log2N, n_rows, rc_max, npad, scale, vis_f32 ? 3 : 0};
     const size_t lds = img_lds_bytes(n_grid, rc_max, r16);
     const int grid = std::min(n_sub * n_steps, 4096);
     if (img_dtype == DSX_F16)
         return r16 ? launch_image<_Float16, true>(P, grid, lds, s) : launch_image<_Float16, false>(P, grid, lds, s);
     return r16 ? launch_image<float, true>(P, grid, lds, s) : launch_image<float, false>(P, grid, lds, s);
+}
+
+extern "C" int dsx_image_subbands(const int32_t *vis, const int32_t *tap_src,
+                                  const int32_t *tap_dst, const int32_t *tap_off, void *img,
+                                  int n_steps, int n_sub, int per_sub, int nbase, int n_grid,
+                                  int n_rows, int npad, float scale, int img_dtype,
+                                  dsx_stream_t stream) {
+    return image_subbands(vis, 0, tap_src, tap_dst, tap_off, img, n_steps, n_sub, per_sub, nbase, n_grid,
+                          n_rows, npad, scale, img_dtype, stream);
+}
+
+extern "C" int dsx_image_subbands_f32(const float *vis, const int32_t *tap_src,
+                                      const int32_t *tap_dst, const int32_t *tap_off, void *img,
+                                      int n_steps, int n_sub, int per_sub, int nbase, int n_grid,
+                                      int n_rows, int npad, float scale, int img_dtype,
+                                      dsx_stream_t stream) {
+    return image_subbands(vis, 1, tap_src, tap_dst, tap_off, img, n_steps, n_sub, per_sub, nbase, n_grid,
+                          n_rows, npad, scale, img_dtype, stream);
 }
 
 extern "C" int dsx_grid_gather(const void *vis, const int32_t *cell_ptr, const int32_t *cell_src,

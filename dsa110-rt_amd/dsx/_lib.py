@@ -28,8 +28,11 @@ PROTOTYPES = {
     "dsx_last_error": [],
     "dsx_unpack_i4": [_P, _P, _P, _I, _I, _I, _I, _F, _P],
     "dsx_corr_i8": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P],
+    "dsx_corr_cal": [_P, _P, _P, _P, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _F, _P],
+    "dsx_apply_cal": [_P, _P, _P, _P, _I, _P, _P, _I, _I, _I, _P],
     "dsx_grid_gather": [_P, _P, _P, _P, _I, _I, _I, _I, _P],
     "dsx_image_subbands": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _F, _I, _P],
+    "dsx_image_subbands_f32": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _F, _I, _P],
     "dsx_dirty_image": [_P, _P, _I, _I, _F, _P],
     "dsx_image_cube": [_P, _P, _I, _I, _I, _I, _I, _I, _F, _P, _P, _P, _I, _P, _I, _P, _I, _P, _P],
     "dsx_ring_advance": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P],

@@ -9,9 +9,15 @@
   The HIP correlator runs int8 MFMA with int32 accumulation; fed int8 planes
   (or raw bytes through :meth:`FastCorrKernel.compute_raw`) it is exact, and
   ``accum_dtype=torch.int32`` returns the integers as ``(..., 2)`` int32.
-  Float planes must hold ``scale * k`` with integer ``k`` in [-8, 7] (what
-  ``unpack_int4_split`` produces); they are mapped back to ``k`` and the
-  result is scaled by ``scale^2``.
+  Float planes (``unpack_int4_split``'s fp16 output, calibrated or not) run
+  the fp16-MFMA correlator with float32 accumulation (``dsx_corr_cal``):
+  the reference's fp16 tensor-core contract; float32 planes are rounded to
+  fp16 operands first.
+* ``apply_cal_split`` / ``make_cal_broadcast_tensors`` / ``compact_cal`` --
+  ``slow_corr_kernel.py:459-569``, ``x2/corr/gemm2x.py:158-168``: complex
+  gains (RFI mask folded in as zeros) on fp16 planes; the hot path applies
+  them on the correlator's operand load instead
+  (:meth:`FastCorrKernel.compute_raw` with ``cal=``).
 * ``stokes_i_pol_sum`` -- ``corr_fast_kernel.py:764-792``.
 """
 from __future__ import annotations
@@ -26,7 +32,8 @@ from . import _lib
 from .constants import LEGACY_FLUFF_SCALE, NPOL, NTIMES_PER_PACKET
 from .pattern import upper_tri_indices
 
-__all__ = ["unpack_int4_split", "FastCorrKernel", "stokes_i_pol_sum", "upper_tri_indices"]
+__all__ = ["unpack_int4_split", "FastCorrKernel", "stokes_i_pol_sum", "upper_tri_indices",
+           "apply_cal_split", "make_cal_broadcast_tensors", "compact_cal"]
 
 
 def unpack_int4_split(raw, *, device="cuda", scale: float = LEGACY_FLUFF_SCALE,
@@ -172,31 +179,146 @@ class FastCorrKernel:
         if n_fv_chunk is not None and not (1 <= n_fv_chunk <= n_fv):
             raise ValueError(f"n_fv_chunk={n_fv_chunk} must be in [1, n_fv_total={n_fv}]")
         if real_v.dtype == torch.int8:
-            re8, im8, scale = real_v.contiguous(), imag_v.contiguous(), 1.0
-        elif real_v.dtype in (torch.float16, torch.float32):
-            nz = torch.cat([real_v.reshape(-1), imag_v.reshape(-1)]).abs()
-            scale = float(nz[nz > 0].min()) if bool((nz > 0).any()) else 1.0
-            k_re = torch.round(real_v.float() / scale)
-            k_im = torch.round(imag_v.float() / scale)
-            if bool((k_re.abs() > 8).any()) or bool((k_im.abs() > 8).any()) or \
-                    not torch.allclose(k_re * scale, real_v.float(), rtol=1e-3, atol=0) or \
-                    not torch.allclose(k_im * scale, imag_v.float(), rtol=1e-3, atol=0):
-                raise ValueError("float voltages must be 4-bit integers times one scale "
-                                 "(as produced by unpack_int4_split)")
-            re8, im8 = k_re.to(torch.int8).contiguous(), k_im.to(torch.int8).contiguous()
-        else:
+            vis = self._run(real_v.contiguous(), imag_v.contiguous(), n_pk, 1, fuse_stokes_i,
+                            chan_sum_factor)
+            return self._finish(vis, 1.0)
+        if real_v.dtype not in (torch.float16, torch.float32):
             raise TypeError(f"voltage dtype {real_v.dtype} not supported")
-        vis = self._run(re8, im8, n_pk, 1, fuse_stokes_i, chan_sum_factor)
-        return self._finish(vis, scale * scale)
+        re16 = real_v.to(torch.float16).contiguous()
+        im16 = imag_v.to(torch.float16).contiguous()
+        return self._run_cal(re16, im16, n_pk, 1, fuse_stokes_i, chan_sum_factor, None, None, 1.0)
+
+    def _run_cal(self, src, im, n_pk, input_layout, fuse_stokes_i, chan_sum_factor, cal, time_keep,
+                 scale):
+        if chan_sum_factor < 1 or self.nchan % chan_sum_factor != 0:
+            raise ValueError(f"chan_sum_factor={chan_sum_factor} must be >= 1 and divide nchan={self.nchan}")
+        if not fuse_stokes_i and chan_sum_factor != 1:
+            raise ValueError("chan_sum_factor > 1 requires fuse_stokes_i")
+        cal_re = cal_im = tk = None
+        n_acc = 1
+        if cal is not None:
+            cal_re, cal_im = _compact_gains(cal, self.nchan, self.nants, self.device)
+        if time_keep is not None:
+            if cal is None:
+                raise ValueError("time_keep needs cal (the keep factors multiply the gains)")
+            tk = torch.as_tensor(time_keep, device=self.device).to(torch.float32).contiguous()
+            if tk.dim() != 3 or tk.shape[1:] != (self.nchan, NPOL) or n_pk % tk.shape[0] != 0:
+                raise ValueError(f"time_keep must be (n_acc, {self.nchan}, {NPOL}) with n_acc "
+                                 f"dividing n_packets ({n_pk}), got {tuple(tk.shape)}")
+            n_acc = int(tk.shape[0])
+        n_fv = n_pk * NTIMES_PER_PACKET // self.t_int_fast_native
+        nco = self.nchan // chan_sum_factor
+        npol = 1 if fuse_stokes_i else 2
+        vis = torch.empty((n_fv, self.nbase, nco, npol, 2), dtype=torch.float32, device=self.device)
+        _lib.require_gpu(src, im, vis, cal_re, cal_im, tk)
+        _lib.call("dsx_corr_cal", _lib.ptr(src), _lib.ptr(im), _lib.ptr(cal_re), _lib.ptr(cal_im),
+                  _lib.ptr(tk), n_acc, _lib.ptr(vis), n_pk, self.nants, self.nchan,
+                  self.t_int_fast_native, chan_sum_factor, int(fuse_stokes_i), input_layout, 0,
+                  float(scale), _lib.stream())
+        v = torch.view_as_complex(vis)
+        return v[:, :, :, 0] if fuse_stokes_i else v
 
     def compute_raw(self, raw: torch.Tensor, *, fuse_stokes_i: bool = True, chan_sum_factor: int = 1,
-                    scale: float = 1.0) -> torch.Tensor:
+                    scale: float = 1.0, cal=None, time_keep=None) -> torch.Tensor:
         """Fused unpack + correlate straight from on-wire bytes
-        ``(n_pkt, nants, nchan, 2, 2)`` uint8 (no voltage planes in HBM)."""
+        ``(n_pkt, nants, nchan, 2, 2)`` uint8 (no voltage planes in HBM).
+
+        Without ``cal``: exact int8 MFMA (result times ``scale**2``).  With
+        ``cal = (cal_real, cal_imag)`` (``make_cal_broadcast_tensors`` /
+        ``compact_cal`` layouts; RFI-masked gains zero) the gains are applied
+        as each voltage ``fp16(code * scale)`` is loaded, rounded once to fp16,
+        and correlated by fp16 MFMA with float32 accumulation
+        (``x2/corr/gemm2x.py:158-168``); ``time_keep`` ``(n_acc, nchan, 2)``
+        multiplies the gains of each accumulation's packets."""
         if raw.dtype != torch.uint8 or raw.dim() != 5 or tuple(raw.shape[1:]) != (self.nants, self.nchan, 2, 2):
             raise ValueError(f"raw must be uint8 (n_pkt, {self.nants}, {self.nchan}, 2, 2)")
         n_pk = int(raw.shape[0])
         if n_pk % (self.t_int_fast_native // NTIMES_PER_PACKET):
             raise ValueError("n_pkt must be a multiple of t_int_fast_native / 2")
+        if cal is not None or time_keep is not None:
+            return self._run_cal(raw.contiguous(), None, n_pk, 0, fuse_stokes_i, chan_sum_factor, cal,
+                                 time_keep, scale)
         vis = self._run(raw.contiguous(), None, n_pk, 0, fuse_stokes_i, chan_sum_factor)
         return self._finish(vis, scale * scale)
+
+
+def _compact_gains(cal, nchan: int, nants: int, device) -> Tuple[torch.Tensor, torch.Tensor]:
+    """``(cal_real, cal_imag)`` in the broadcast layout ``(NCHAN or 1, 1, NPOL, 1,
+    NANTS)`` (``make_cal_broadcast_tensors``) or the compact ``(NCHAN, NPOL,
+    NANTS)`` (``compact_cal``) -> float32 ``(nchan, 2, nants)`` on ``device``."""
+    if not (isinstance(cal, (tuple, list)) and len(cal) == 2):
+        raise TypeError("cal must be a (cal_real, cal_imag) pair")
+    out = []
+    for c in cal:
+        c = torch.as_tensor(c, device=device)
+        if c.is_complex():
+            raise TypeError("cal_real / cal_imag must be real tensors")
+        if c.dim() == 5:
+            if c.shape[1] != 1 or c.shape[3] != 1:
+                raise ValueError(f"broadcast gains must be (NCHAN or 1, 1, NPOL, 1, NANTS), got {tuple(c.shape)}")
+            c = c[:, 0, :, 0, :]
+        if c.dim() != 3 or c.shape[1:] != (NPOL, nants) or c.shape[0] not in (1, nchan):
+            raise ValueError(f"gains must be ({nchan} or 1, {NPOL}, {nants}), got {tuple(c.shape)}")
+        out.append(c.to(torch.float32).expand(nchan, NPOL, nants).contiguous())
+    return out[0], out[1]
+
+
+def make_cal_broadcast_tensors(gains_fine, *, device="cpu", dtype: torch.dtype = torch.float16):
+    """``(NANTS, NCHAN, NPOL)`` complex gains -> ``(cal_real, cal_imag)`` each
+    ``(NCHAN, 1, NPOL, 1, NANTS)`` (``slow_corr_kernel.py:572-611``)."""
+    g = np.asarray(gains_fine)
+    if g.ndim != 3 or g.shape[2] != NPOL:
+        raise ValueError(f"gains_fine shape {g.shape}, expected (NANTS, NCHAN, {NPOL})")
+    real = torch.from_numpy(np.ascontiguousarray(g.real)).to(device=device, dtype=dtype)
+    imag = torch.from_numpy(np.ascontiguousarray(g.imag)).to(device=device, dtype=dtype)
+    real = real.permute(1, 2, 0).contiguous()[:, None, :, None, :]
+    imag = imag.permute(1, 2, 0).contiguous()[:, None, :, None, :]
+    return real, imag
+
+
+def compact_cal(cal_real: torch.Tensor, cal_imag: torch.Tensor, cube_mask: Optional[torch.Tensor] = None):
+    """Broadcast gains ``(NCHAN, 1, NPOL, 1, NANTS)`` -> float32 ``(NCHAN, NPOL,
+    NANTS)`` with the RFI mask ``cube_mask`` ``(ANT, CH, POL)`` (True = flagged)
+    folded in as zero gains (``x2/corr/gemm2x.py:158-168``)."""
+    cr = cal_real[:, 0, :, 0, :].float()
+    ci = cal_imag[:, 0, :, 0, :].float()
+    if cube_mask is not None:
+        keep = (~torch.as_tensor(cube_mask, device=cr.device, dtype=torch.bool)).permute(1, 2, 0).to(cr.dtype)
+        cr = cr * keep
+        ci = ci * keep
+    return cr.contiguous(), ci.contiguous()
+
+
+def apply_cal_split(real_v: torch.Tensor, imag_v: torch.Tensor, cal_real_fine: torch.Tensor,
+                    cal_imag_fine: torch.Tensor):
+    """Per-(ant, fine_ch, pol) complex gain on split fp16 voltages in GEMM layout
+    ``(NCHAN, 2, NPOL, NPKT, NANTS)`` (``slow_corr_kernel.py:459-569``):
+    ``real_out = fp16(v_re g_re - v_im g_im)``, ``imag_out = fp16(v_re g_im + v_im g_re)``,
+    each product and sum rounded in float32, one fp16 rounding.  Gains are
+    ``(NCHAN or 1, 1, NPOL, 1, NANTS)`` of the voltages' dtype.  Returns new
+    tensors; raises ``ValueError`` on a shape / dtype / device mismatch."""
+    if real_v.shape != imag_v.shape:
+        raise ValueError(f"real_v shape {tuple(real_v.shape)} != imag_v shape {tuple(imag_v.shape)}")
+    if real_v.dim() != 5 or real_v.shape[1] != NTIMES_PER_PACKET or real_v.shape[2] != NPOL:
+        raise ValueError(f"voltage shape {tuple(real_v.shape)} is not (NCHAN, {NTIMES_PER_PACKET}, "
+                         f"{NPOL}, NPKT, NANTS)")
+    if real_v.dtype != imag_v.dtype:
+        raise ValueError(f"real_v dtype {real_v.dtype} != imag_v dtype {imag_v.dtype}")
+    if cal_real_fine.dtype != real_v.dtype or cal_imag_fine.dtype != real_v.dtype:
+        raise ValueError(f"cal dtypes ({cal_real_fine.dtype}, {cal_imag_fine.dtype}) != "
+                         f"voltage dtype {real_v.dtype}")
+    if cal_real_fine.device != real_v.device or cal_imag_fine.device != real_v.device:
+        raise ValueError(f"cal devices ({cal_real_fine.device}, {cal_imag_fine.device}) != "
+                         f"voltage device {real_v.device}")
+    if real_v.dtype != torch.float16:
+        raise TypeError(f"apply_cal_split runs on fp16 voltages, got {real_v.dtype}")
+    nch, _, _, npk, na = real_v.shape
+    cr, ci = _compact_gains((cal_real_fine, cal_imag_fine), nch, na, real_v.device)
+    re = real_v.contiguous()
+    im = imag_v.contiguous()
+    out_re = torch.empty_like(re)
+    out_im = torch.empty_like(im)
+    _lib.require_gpu(re, im, cr, ci, out_re, out_im)
+    _lib.call("dsx_apply_cal", _lib.ptr(re), _lib.ptr(im), _lib.ptr(cr), _lib.ptr(ci), 1,
+              _lib.ptr(out_re), _lib.ptr(out_im), nch, npk, na, _lib.stream())
+    return out_re, out_im

@@ -154,6 +154,7 @@ class SearchNode:
         self.lds_budget = int(lds_budget)
         self.dedisp_mode = int(dedisp_mode)
         self.plans: Dict[int, tuple] = {}
+        self.cal = None                                  # (gains re, im, time keep, scale) when calibrating
         self.T0 = 0
         self.prev_len = 0
         self.marks: Optional[Dict[str, list]] = None     # stage -> [start, end] events when timing
@@ -241,10 +242,49 @@ class SearchNode:
             raise ValueError(f"{n_steps} steps > max_block_steps {self.cap}")
         vis = self.vis_slots[slot][:n_steps]
         self._mark("corr", 0)
-        _lib.call("dsx_corr_i8", _lib.ptr(raw), None, _lib.ptr(vis), n_pkt, cfg.n_ant, cfg.n_chan,
-                  cfg.t_int, cfg.csf, 1, 0, 1, _lib.stream(self._stream(stream)))
+        if self.cal is None:
+            _lib.call("dsx_corr_i8", _lib.ptr(raw), None, _lib.ptr(vis), n_pkt, cfg.n_ant, cfg.n_chan,
+                      cfg.t_int, cfg.csf, 1, 0, 1, _lib.stream(self._stream(stream)))
+        else:
+            # calibrated: float (re, im) pairs in the same slot
+            vis = vis.view(torch.float32)
+            cr, ci, tk, scale = self.cal
+            n_acc = int(tk.shape[0]) if tk is not None else 1
+            if tk is not None and n_pkt % n_acc:
+                raise ValueError(f"time_keep has {n_acc} accumulations; {n_pkt} packets do not divide")
+            _lib.call("dsx_corr_cal", _lib.ptr(raw), None, _lib.ptr(cr), _lib.ptr(ci), _lib.ptr(tk), n_acc,
+                      _lib.ptr(vis), n_pkt, cfg.n_ant, cfg.n_chan, cfg.t_int, cfg.csf, 1, 0, 1, float(scale),
+                      _lib.stream(self._stream(stream)))
         self._mark("corr", 1)
         return vis
+
+    def set_calibration(self, cal_real=None, cal_imag=None, *, cube_mask=None, time_keep=None,
+                        scale: float = 1.0) -> None:
+        """Apply complex gains on the correlator's operand load from now on.
+
+        ``cal_real``/``cal_imag``: ``(NCHAN or 1, 1, 2, 1, NANTS)``
+        (``make_cal_broadcast_tensors``) or ``(NCHAN, 2, NANTS)``; ``cube_mask``
+        ``(ANT, CH, POL)`` True = flagged (zero gain); ``time_keep`` ``(n_acc,
+        NCHAN, 2)`` per-accumulation keep factors of every block; voltages are
+        ``fp16(code * scale)``.  Visibilities become float32 (fp16 MFMA, float32
+        accumulation) and the imager sums them with float32 LDS atomics.
+        ``set_calibration()`` with no gains returns to the exact integer path."""
+        from .corr import _compact_gains
+        cfg = self.cfg
+        if cal_real is None:
+            self.cal = None
+            return
+        cr, ci = _compact_gains((cal_real, cal_imag), cfg.n_chan, cfg.n_ant, self.dev)
+        if cube_mask is not None:
+            keep = (~torch.as_tensor(cube_mask, device=self.dev, dtype=torch.bool)).permute(1, 2, 0)
+            cr = (cr * keep).contiguous()
+            ci = (ci * keep).contiguous()
+        tk = None
+        if time_keep is not None:
+            tk = torch.as_tensor(time_keep, device=self.dev).to(torch.float32).contiguous()
+            if tk.dim() != 3 or tuple(tk.shape[1:]) != (cfg.n_chan, 2):
+                raise ValueError(f"time_keep must be (n_acc, {cfg.n_chan}, 2), got {tuple(tk.shape)}")
+        self.cal = (cr, ci, tk, float(scale))
 
     def image(self, vis: torch.Tensor, n_steps: int, slot: int = 0,
               stream: Optional[torch.cuda.Stream] = None) -> torch.Tensor:
@@ -252,7 +292,7 @@ class SearchNode:
         N = cfg.n_grid
         img = self.img_slots[slot][:cfg.n_sub * n_steps * N * N].view(cfg.n_sub, n_steps, N, N)
         self._mark("image", 0)
-        _lib.call("dsx_image_subbands", _lib.ptr(vis), _lib.ptr(self.tap_src), _lib.ptr(self.tap_dst),
+        _lib.call("dsx_image_subbands" if vis.dtype == torch.int32 else "dsx_image_subbands_f32", _lib.ptr(vis), _lib.ptr(self.tap_src), _lib.ptr(self.tap_dst),
                   _lib.ptr(self.tap_off), _lib.ptr(img), n_steps, cfg.n_sub, cfg.per_sub, cfg.nbase,
                   cfg.n_grid, self.n_rows, self.npad, 1.0,
                   _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,

@@ -73,6 +73,38 @@ int dsx_corr_i8(const void *raw, const void *im, int32_t *vis, int n_pkt, int n_
                 int n_chan, int t_int, int csf, int fuse_pol, int input_layout,
                 int out_layout, dsx_stream_t stream);
 
+/* ----------------------------------------------- calibrated correlator
+ * Complex gains (and the RFI mask, folded in as zero gains) applied to every
+ * voltage as it is loaded; fp16 MFMA with float32 accumulation.
+ *   input_layout 0: volt_re holds on-wire bytes (n_pkt, n_ant, n_chan, 2, 2),
+ *                   volt_im NULL; voltage v = fp16(code * scale).
+ *   input_layout 1: volt_re / volt_im hold fp16 planes in GEMM layout
+ *                   (n_chan, 2, 2, n_pkt, n_ant) (e.g. unpack_int4_split output).
+ * cal_re / cal_im: float32 (n_chan, 2 pol, n_ant) gains, or both NULL (the
+ * operands are the voltages).  time_keep: NULL or float32 (n_acc, n_chan,
+ * 2 pol) factors multiplying the gain of every packet of accumulation
+ * pkt / (n_pkt / n_acc) (n_acc divides n_pkt).  Operand
+ * x = fp16(v_re g_re - v_im g_im, v_re g_im + v_im g_re), each product and
+ * sum rounded in float32 (no fused multiply-add).  vis: float (re, im) pairs,
+ * vis[bls] = sum conj(x_b) x_a, in the layouts of dsx_corr_i8 (out_layout
+ * 0 / 1).  n_ant <= 128.
+ * Replaces apply_cal_split + FastCorrKernel.compute_split
+ * (src/dsart/services/slow_corr_kernel.py:459-569,
+ * services/corr_fast_kernel.py:364-499) and the cal-on-load GEMM
+ * src/dsart/x2/corr/gemm2x.py:158-168 (CPU twin gemm2x.py:346-390). */
+int dsx_corr_cal(const void *volt_re, const void *volt_im, const float *cal_re,
+                 const float *cal_im, const float *time_keep, int n_acc, float *vis,
+                 int n_pkt, int n_ant, int n_chan, int t_int, int csf, int fuse_pol,
+                 int input_layout, int out_layout, float scale, dsx_stream_t stream);
+
+/* fp16 planes (n_chan, 2, 2, n_pkt, n_ant) times float32 gains
+ * (n_chan, 2, n_ant) (cal_per_chan != 0) or (2, n_ant) (one gain for every
+ * channel): out = fp16(v * g) with the operand arithmetic of dsx_corr_cal.
+ * Replaces apply_cal_split, src/dsart/services/slow_corr_kernel.py:459-569. */
+int dsx_apply_cal(const void *re, const void *im, const float *cal_re, const float *cal_im,
+                  int cal_per_chan, void *out_re, void *out_im, int n_chan, int n_pkt,
+                  int n_ant, dsx_stream_t stream);
+
 /* --------------------------------------------------------------- gridder
  * Gather form of the nearest-cell gridder: for every step f and filled cell
  * c, out[f][c] = sum over sources i in [cell_ptr[c], cell_ptr[c+1]) of
@@ -103,6 +135,17 @@ int dsx_image_subbands(const int32_t *vis, const int32_t *tap_src, const int32_t
                        const int32_t *tap_off, void *img, int n_steps, int n_sub,
                        int per_sub, int nbase, int n_grid, int n_rows, int npad,
                        float scale, int img_dtype, dsx_stream_t stream);
+
+/* dsx_image_subbands for float32 (re, im) visibilities (the calibrated
+ * correlator's output, dsx_corr_cal out_layout 1): the half plane is summed
+ * with float32 LDS atomics, so the order of a cell's additions (and its last
+ * bits) is not fixed from run to run.
+ * Replaces the same reference functions as dsx_image_subbands, fed the
+ * calibrated visibilities of apply_cal_split (slow_corr_kernel.py:459-569). */
+int dsx_image_subbands_f32(const float *vis, const int32_t *tap_src, const int32_t *tap_dst,
+                           const int32_t *tap_off, void *img, int n_steps, int n_sub,
+                           int per_sub, int nbase, int n_grid, int n_rows, int npad,
+                           float scale, int img_dtype, dsx_stream_t stream);
 
 /* Fused dequant + subband shift-and-add + C2R imager for a search cube.
  * For every trial f < n_fdm and cube time t in [t_lo, t_det):

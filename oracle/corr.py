@@ -6,6 +6,12 @@ known answer ``E_1 = (1, 0)``, ``E_2 = (0, 1)`` -> ``V = +iK``
 (``tests/test_slow_corr_synth.py:219-277``), zero-in/zero-out, real
 non-negative autos, and an independent complex-loop brute force on tiny
 inputs -- see ``tests/test_oracle_corr.py``.
+
+The calibrated operands (:func:`cal_operands`, :func:`correlate_float`) are
+pinned by a known answer with gains exactly representable in fp16
+(``V_cal = conj(g_b) g_a V``, the gain application of
+``slow_corr_kernel.py:459-502``) and by masked gains / zero keep factors
+equalling zeroed voltages (``x2/corr/gemm2x.py:158-168,373-378``).
 """
 from __future__ import annotations
 
@@ -105,3 +111,67 @@ def correlate_blas(raw: np.ndarray, *, n_pkt: int, n_ant: int, n_chan: int, t_in
     a_idx, b_idx = upper_tri(n_ant)
     out = np.stack([vre[..., a_idx, b_idx], vim[..., a_idx, b_idx]], axis=-1)
     return out.astype(np.int64)
+
+
+# ---------------------------------------------------------------------------
+# Calibrated correlator (cal + RFI mask on operand load)
+# ---------------------------------------------------------------------------
+
+def cal_operands(raw: np.ndarray, *, n_pkt: int, n_ant: int, n_chan: int, scale: float,
+                 cal_re=None, cal_im=None, time_keep=None) -> Tuple[np.ndarray, np.ndarray]:
+    """fp16 operands of the calibrated correlator, as float64
+    ``(sample, ant, chan, pol)`` arrays.
+
+    Restates the operand arithmetic of ``gemm2x_reference``
+    (``src/dsart/x2/corr/gemm2x.py:346-390``): the unpacked voltage
+    ``v = fp16(code * scale)`` (``unpack_int4_split``'s fp16 output,
+    ``services/slow_corr_kernel.py:220-403``) times the per-(chan, pol, ant)
+    complex gain ``g`` in float32 (products and sum rounded separately, no
+    fused multiply-add), rounded once to fp16:
+    ``x_re = fp16(v_re g_re - v_im g_im)``, ``x_im = fp16(v_re g_im + v_im g_re)``.
+    ``cal_re``/``cal_im`` are ``(n_chan, 2, n_ant)`` (``compact_cal``'s layout,
+    RFI-masked entries zero); ``time_keep`` ``(n_acc, n_chan, 2)`` multiplies the
+    gain of every packet of accumulation ``pkt // (n_pkt // n_acc)``
+    (``gemm2x.py:373-378``).  Without a gain the operands are the fp16
+    voltages themselves.
+    """
+    re, im = voltages(raw, n_pkt, n_ant, n_chan)                  # int64 (S, ant, chan, pol)
+    s32 = np.float32(scale)
+    vr = (re.astype(np.float32) * s32).astype(np.float16).astype(np.float32)
+    vi = (im.astype(np.float32) * s32).astype(np.float16).astype(np.float32)
+    if cal_re is None:
+        return vr.astype(np.float64), vi.astype(np.float64)
+    gr = np.asarray(cal_re, np.float32).transpose(2, 0, 1)[None]     # (1, ant, chan, pol)
+    gi = np.asarray(cal_im, np.float32).transpose(2, 0, 1)[None]
+    if time_keep is not None:
+        tk = np.asarray(time_keep, np.float32)
+        n_acc = tk.shape[0]
+        acc = (np.arange(2 * n_pkt) // 2) // (n_pkt // n_acc)       # accumulation of each sample
+        tk = tk[acc][:, None, :, :]                                   # (S, 1, chan, pol)
+        gr = gr * tk
+        gi = gi * tk
+    xr = ((vr * gr) - (vi * gi)).astype(np.float16)
+    xi = ((vr * gi) + (vi * gr)).astype(np.float16)
+    return xr.astype(np.float64), xi.astype(np.float64)
+
+
+def correlate_float(xr: np.ndarray, xi: np.ndarray, *, t_int: int, csf: int = 1,
+                    fuse_pol: bool = True) -> Tuple[np.ndarray, np.ndarray]:
+    """float64 visibilities ``V[bls] = sum conj(x_b) x_a`` of float operands
+    ``(sample, ant, chan, pol)`` (``gemm2x_reference``'s float64
+    accumulation), and the bound ``sum |x_b| |x_a|`` of each output that a
+    float32-accumulated result is compared against.  Layout as
+    :func:`correlate`, complex128: ``(n_steps, n_chan // csf, nbase)`` or
+    ``(n_steps, n_chan // csf, 2, nbase)``."""
+    S, n_ant, n_chan, _ = xr.shape
+    n_steps = S // t_int
+    nco = n_chan // csf
+    R = xr.reshape(n_steps, t_int, n_ant, nco, csf, 2).transpose(0, 1, 3, 4, 5, 2)
+    I = xi.reshape(n_steps, t_int, n_ant, nco, csf, 2).transpose(0, 1, 3, 4, 5, 2)
+    sub = "stcxpa,stcxpb->scab" if fuse_pol else "stcxpa,stcxpb->scpab"
+    vre = np.einsum(sub, R, R) + np.einsum(sub, I, I)
+    vim = np.einsum(sub, I, R) - np.einsum(sub, R, I)
+    bound = np.einsum(sub, np.abs(R) + np.abs(I), np.abs(R) + np.abs(I))
+    a_idx, b_idx = upper_tri(n_ant)
+    v = vre[..., a_idx, b_idx] + 1j * vim[..., a_idx, b_idx]
+    return v, bound[..., a_idx, b_idx]

@@ -126,3 +126,68 @@ def test_blas_twin_equals_einsum_form():
     raw8 = np.full((n_pkt, n_ant, n_chan, 2, 2), 0x88, dtype=np.uint8)
     assert np.array_equal(ocorr.correlate_blas(raw8, n_pkt=n_pkt, n_ant=n_ant, n_chan=n_chan, t_int=8, csf=4),
                           ocorr.correlate(raw8, n_pkt=n_pkt, n_ant=n_ant, n_chan=n_chan, t_int=8, csf=4))
+
+
+# ---- calibrated correlator (cal + RFI mask on operand load) ----
+
+def _rand_raw(rng, S, A, C):
+    re = np.clip(np.rint(rng.normal(0, 2.5, size=(S, A, C, 2))), -7, 7).astype(np.int64)
+    im = np.clip(np.rint(rng.normal(0, 2.5, size=(S, A, C, 2))), -7, 7).astype(np.int64)
+    return re, im, _pack(re, im)
+
+
+def test_cal_operands_without_gains_are_fp16_voltages():
+    rng = np.random.default_rng(11)
+    re, im, raw = _rand_raw(rng, 16, 4, 3)
+    xr, xi = ocorr.cal_operands(raw, n_pkt=8, n_ant=4, n_chan=3, scale=0.05)
+    assert np.array_equal(xr, (re * np.float32(0.05)).astype(np.float32).astype(np.float16).astype(np.float64))
+    xr1, xi1 = ocorr.cal_operands(raw, n_pkt=8, n_ant=4, n_chan=3, scale=1.0)
+    assert np.array_equal(xr1, re) and np.array_equal(xi1, im)
+
+
+def test_cal_known_answer_gain_product():
+    """Gains exactly representable in fp16 (and their products with 4-bit
+    codes): V_cal[a, b] = conj(g_b) g_a V[a, b] exactly (the CASA-style G
+    application of slow_corr_kernel.py:459-502)."""
+    rng = np.random.default_rng(12)
+    S, A, C, t_int = 16, 5, 2, 8
+    re, im, raw = _rand_raw(rng, S, A, C)
+    gvals = np.array([1.0, 0.5, -0.25 + 0.5j, 0.75j, 1.5 - 0.25j])
+    g = np.broadcast_to(gvals[:, None, None], (A, C, 2))           # (ant, chan, pol)
+    cal_re = np.ascontiguousarray(g.real.transpose(1, 2, 0))       # (chan, pol, ant)
+    cal_im = np.ascontiguousarray(g.imag.transpose(1, 2, 0))
+    xr, xi = ocorr.cal_operands(raw, n_pkt=S // 2, n_ant=A, n_chan=C, scale=1.0, cal_re=cal_re, cal_im=cal_im)
+    # operands are exact: x = g * E
+    assert np.array_equal(xr + 1j * xi, (re + 1j * im) * gvals[None, :, None, None])
+    v, bound = ocorr.correlate_float(xr, xi, t_int=t_int)
+    v_int = ocorr.correlate(raw, n_pkt=S // 2, n_ant=A, n_chan=C, t_int=t_int)
+    v_int = v_int[..., 0] + 1j * v_int[..., 1]
+    a, b = ocorr.upper_tri(A)
+    want = np.conj(gvals[b]) * gvals[a] * v_int
+    assert np.array_equal(v, want)
+    assert np.all(bound >= np.abs(v))
+
+
+def test_cal_rfi_mask_and_time_keep_zero_their_samples():
+    """A zero gain (masked (ant, chan, pol)) equals zeroing those voltages;
+    a zero keep factor removes every packet of its accumulation."""
+    rng = np.random.default_rng(13)
+    S, A, C, t_int = 32, 4, 3, 8
+    re, im, raw = _rand_raw(rng, S, A, C)
+    cal_re = np.ones((C, 2, A), np.float32)
+    cal_im = np.zeros((C, 2, A), np.float32)
+    cal_re[1, 0, 2] = 0.0                         # mask ant 2, chan 1, pol 0
+    keep = np.ones((4, C, 2), np.float32)
+    keep[2, 0, 1] = 0.0                           # accumulation 2 (packets 8..11), chan 0, pol 1
+    xr, xi = ocorr.cal_operands(raw, n_pkt=S // 2, n_ant=A, n_chan=C, scale=1.0, cal_re=cal_re,
+                                cal_im=cal_im, time_keep=keep)
+    re2, im2 = re.copy(), im.copy()
+    re2[:, 2, 1, 0] = 0
+    im2[:, 2, 1, 0] = 0
+    re2[16:24, :, 0, 1] = 0                       # samples of packets 8..11
+    im2[16:24, :, 0, 1] = 0
+    assert np.array_equal(xr, re2) and np.array_equal(xi, im2)
+    v, _ = ocorr.correlate_float(xr, xi, t_int=t_int, fuse_pol=False)
+    w = ocorr.correlate(_pack(re2, im2), n_pkt=S // 2, n_ant=A, n_chan=C, t_int=t_int, fuse_pol=False)
+    assert np.array_equal(v, w[..., 0] + 1j * w[..., 1])
+
