@@ -30,6 +30,12 @@ struct CorrParams {
     int n_pkt, n_ant, n_chan, t_int, csf, fuse_pol, in_layout, out_layout;
     int vec4;   // raw input read 4 channels (16 bytes) at a time: csf, n_chan multiples of 4
     int n_steps, n_cout, nbase, kq, row_stride, n_tiles, n_pairs, n_items;
+    // stage 1: output step t reads channel c at step t + chan_shift[c] - delay of the
+    // raw block, steps before the block from `tail` (the previous block's last
+    // `delay` steps); NULL chan_shift: no stage 1
+    const int32_t *chan_shift;
+    const uint8_t *tail;
+    int delay;
 };
 
 __device__ __forceinline__ void store_vis(const CorrParams &P, int step, int co, int q, int bls,
@@ -121,8 +127,23 @@ __global__ __launch_bounds__(CORR_THREADS) void corr_i8_kernel(CorrParams P) {
                 const int pkt_l = rest / P.n_ant;
                 const int pkt = step * ppf + pkt_l;
                 const int ch = co * P.csf + ch_l;
-                const uint32_t word = *reinterpret_cast<const uint32_t *>(
-                    P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + ch) << 2));
+                uint32_t word;
+                if (P.chan_shift == nullptr) {
+                    word = *reinterpret_cast<const uint32_t *>(
+                        P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + ch) << 2));
+                } else {
+                    // stage 1: this channel's samples of step t + shift - delay
+                    const int64_t src = (int64_t)(step + P.chan_shift[ch] - P.delay) * ppf + pkt_l;
+                    const int64_t n_tail = (int64_t)P.delay * ppf;
+                    if (src >= 0 && src < P.n_pkt)
+                        word = *reinterpret_cast<const uint32_t *>(
+                            P.raw + (((src * P.n_ant + ant) * P.n_chan + ch) << 2));
+                    else if (src < 0 && src >= -n_tail)
+                        word = *reinterpret_cast<const uint32_t *>(
+                            P.tail + ((((src + n_tail) * P.n_ant + ant) * P.n_chan + ch) << 2));
+                    else
+                        word = 0;    // outside both (the host keeps shifts in [0, delay])
+                }
                 // bytes: (t0 p0, t0 p1, t1 p0, t1 p1)
                 const uint32_t r = sext_nib4(word & 0x0F0F0F0Fu);
                 const uint32_t m = sext_nib4((word >> 4) & 0x0F0F0F0Fu);
@@ -260,9 +281,9 @@ extern "C" int dsx_unpack_i4(const uint8_t *raw, void *re, void *im, int n_pkt, 
     return check_launch("dsx_unpack_i4");
 }
 
-extern "C" int dsx_corr_i8(const void *raw, const void *im, int32_t *vis, int n_pkt, int n_ant,
-                           int n_chan, int t_int, int csf, int fuse_pol, int input_layout,
-                           int out_layout, dsx_stream_t stream) {
+static int corr_i8(const void *raw, const void *im, const uint8_t *tail, const int32_t *chan_shift, int delay,
+                   int32_t *vis, int n_pkt, int n_ant, int n_chan, int t_int, int csf, int fuse_pol,
+                   int input_layout, int out_layout, dsx_stream_t stream) {
     DSX_REQUIRE(raw && vis, "dsx_corr_i8: null pointer");
     DSX_REQUIRE(input_layout == 0 || input_layout == 1, "dsx_corr_i8: input_layout must be 0 or 1");
     DSX_REQUIRE(input_layout == 0 ? im == nullptr : im != nullptr,
@@ -286,7 +307,11 @@ extern "C" int dsx_corr_i8(const void *raw, const void *im, int32_t *vis, int n_
     P.fuse_pol = fuse_pol ? 1 : 0;
     P.in_layout = input_layout;
     P.out_layout = out_layout;
-    P.vec4 = input_layout == 0 && csf % 4 == 0 && n_chan % 4 == 0 && ((uintptr_t)raw & 15) == 0;
+    P.vec4 = input_layout == 0 && chan_shift == nullptr && csf % 4 == 0 && n_chan % 4 == 0 &&
+             ((uintptr_t)raw & 15) == 0;
+    P.chan_shift = chan_shift;
+    P.tail = tail;
+    P.delay = delay;
     P.n_steps = 2 * n_pkt / t_int;
     P.n_cout = n_chan / csf;
     P.nbase = n_ant * (n_ant + 1) / 2;
@@ -303,4 +328,23 @@ extern "C" int dsx_corr_i8(const void *raw, const void *im, int32_t *vis, int n_
                               (int)lds);
     corr_i8_kernel<<<grid, CORR_THREADS, lds, as_stream(stream)>>>(P);
     return check_launch("dsx_corr_i8");
+}
+
+extern "C" int dsx_corr_i8(const void *raw, const void *im, int32_t *vis, int n_pkt, int n_ant,
+                           int n_chan, int t_int, int csf, int fuse_pol, int input_layout,
+                           int out_layout, dsx_stream_t stream) {
+    return corr_i8(raw, im, nullptr, nullptr, 0, vis, n_pkt, n_ant, n_chan, t_int, csf, fuse_pol,
+                   input_layout, out_layout, stream);
+}
+
+extern "C" int dsx_corr_i8_stage1(const uint8_t *raw, const uint8_t *tail, const int32_t *chan_shift,
+                                  int delay, int32_t *vis, int n_pkt, int n_ant, int n_chan, int t_int,
+                                  int csf, int fuse_pol, int out_layout, dsx_stream_t stream) {
+    DSX_REQUIRE(raw && chan_shift && vis, "dsx_corr_i8_stage1: null pointer");
+    DSX_REQUIRE(delay >= 0 && (delay == 0 || tail != nullptr), "dsx_corr_i8_stage1: delay %d needs a tail",
+                delay);
+    DSX_REQUIRE(t_int >= 2 && t_int % 2 == 0 && (int64_t)delay * (t_int / 2) <= n_pkt,
+                "dsx_corr_i8_stage1: the %d-step delay exceeds the block", delay);
+    return corr_i8(raw, nullptr, tail, chan_shift, delay, vis, n_pkt, n_ant, n_chan, t_int, csf, fuse_pol, 0,
+                   out_layout, stream);
 }

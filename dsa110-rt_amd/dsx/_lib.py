@@ -28,6 +28,8 @@ PROTOTYPES = {
     "dsx_last_error": [],
     "dsx_unpack_i4": [_P, _P, _P, _I, _I, _I, _I, _F, _P],
     "dsx_corr_i8": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P],
+    "dsx_corr_i8_stage1": [_P, _P, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _P],
+    "dsx_static_sky": [_P, _I, _I, _I, _I64, _P, _P, _I, _I, _I, _I, _P],
     "dsx_corr_cal": [_P, _P, _P, _P, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _F, _P],
     "dsx_apply_cal": [_P, _P, _P, _P, _I, _P, _P, _I, _I, _I, _P],
     "dsx_grid_gather": [_P, _P, _P, _P, _I, _I, _I, _I, _P],

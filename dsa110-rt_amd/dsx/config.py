@@ -48,6 +48,22 @@ class NodeConfig:
     dm_endpoint: bool = True               # trials include dm_max
     # uneven plan: ((dm_lo, dm_hi, n_dm, time downsampling, endpoint), ...); empty = one range
     dm_ranges: Tuple[Tuple[float, float, int, int, bool], ...] = ()
+    #: explicit DM trials (overrides dm_min / dm_max / n_dm spacing when non-empty)
+    dm_list: Tuple[float, ...] = ()
+    # ---- stage 1 (per-channel coarse-DM shifts before gridding)
+    #: coarse DM whose delay inside each subband is removed before gridding
+    #: (None: no stage 1); each group of ``stage1_csf`` channels is read
+    #: ``stage1_bins`` steps later, aligning it to its subband's first group
+    stage1_dm: Optional[float] = None
+    stage1_csf: int = 1
+    #: steps the stage-1 stream trails the raw stream (0: the largest stage-1 bin)
+    stage1_delay_steps: int = 0
+    #: coarse DM trials of a multi-coarse-DM node: every fine trial is searched on
+    #: the stage-1 stream of its nearest coarse DM
+    coarse_dms: Tuple[float, ...] = ()
+    # ---- static-sky subtraction (per-pixel mean of the previous blocks)
+    static_sky_blocks: int = 0             # window in blocks (0: off)
+    static_sky_warmup: int = 8             # blocks passed through before subtracting
     widths: Tuple[int, ...] = (1, 2, 4, 8)
     threshold: float = 8.0
     min_stat_count: int = 512              # complete samples per DM before it is scored
@@ -119,11 +135,78 @@ class NodeConfig:
 
     @property
     def subband_ref_GHz(self) -> np.ndarray:
-        """Reference frequency of each subband: its first (highest) output channel.
+        """Reference frequency of each subband: its first (highest) output channel,
+        or with stage 1 its first stage-1 channel group (the frequency stage 1
+        aligns the subband to).
 
         Same rule as ``coarse_dm/dm_plan.py:179-191``.
         """
+        if self.stage1_dm is not None:
+            f = self.stage1_freqs_GHz
+            return np.ascontiguousarray(f[:: f.size // self.n_sub][: self.n_sub])
         return np.ascontiguousarray(self.freq_out_GHz[:: self.per_sub][: self.n_sub])
+
+    @property
+    def stage1_freqs_GHz(self) -> np.ndarray:
+        """Mean frequency of each group of ``stage1_csf`` channels (the summed
+        channel table of ``coarse_dm/dm_plan.py:160-176``)."""
+        f = np.asarray(self.chan_freqs_GHz, dtype=np.float64)
+        if self.stage1_csf == 1:
+            return f
+        return f.reshape(-1, self.stage1_csf).mean(axis=1)
+
+    def stage1_bins_for(self, dm: float) -> np.ndarray:
+        """``(n_chan,)`` int32 stage-1 shift of every channel for coarse DM ``dm``
+        (``DMPlan.delay_bins_per_subband``, ``coarse_dm/dm_plan.py:479-515``):
+        ``rint(rint(K dm (nu_j^-2 - nu_ref^-2) 1e3 / 32.768 us) / t_int)`` for the
+        channel's group ``j``, ``nu_ref`` the first group of its subband; two
+        roundings (native sample, then step), every subband's first group 0."""
+        from .constants import K_DM_MS_GHZ2_PC
+        f = self.stage1_freqs_GHz
+        per = f.size // self.n_sub
+        ref = np.repeat(f[::per][: self.n_sub], per)
+        inv_diff = (1.0 / f ** 2) - (1.0 / ref ** 2)
+        delay_us = K_DM_MS_GHZ2_PC * float(dm) * inv_diff * 1e3
+        nat = np.rint(delay_us / NATIVE_SAMPLE_US)
+        bins = np.rint(nat / self.t_int).astype(np.int32)
+        return np.ascontiguousarray(np.repeat(bins, self.stage1_csf))
+
+    @property
+    def stage1_bins(self) -> np.ndarray:
+        if self.stage1_dm is None:
+            return np.zeros(self.n_chan, np.int32)
+        return self.stage1_bins_for(self.stage1_dm)
+
+    @property
+    def stage1_delay(self) -> int:
+        """Steps the stage-1 stream trails the raw stream: output step ``t`` of
+        a block reads channel ``c`` at ``t + bins[c]``, so a block emits the
+        steps ``[T0 - M, T0 + L - M)`` and keeps its last ``M`` raw steps."""
+        if self.stage1_dm is None:
+            return 0
+        return max(int(self.stage1_delay_steps), int(self.stage1_bins.max()))
+
+    def coarse_groups(self) -> Tuple["NodeConfig", ...]:
+        """One stage-1 configuration per coarse DM of a multi-coarse-DM node:
+        the fine trials whose nearest coarse DM it is (ties to the lower one),
+        all groups delayed by the largest stage-1 bin of any coarse DM."""
+        coarse = np.asarray(self.coarse_dms, dtype=np.float64)
+        fine = self.dm_trials
+        f2c = np.argmin(np.abs(fine[:, None] - coarse[None, :]), axis=1)
+        M = max(int(self.stage1_bins_for(float(c)).max()) for c in coarse)
+        out = []
+        for k, c in enumerate(coarse):
+            sel = fine[f2c == k]
+            if sel.size == 0:
+                continue
+            out.append(replace(self, coarse_dms=(), dm_ranges=(), dm_list=tuple(float(x) for x in sel),
+                               n_dm=int(sel.size), stage1_dm=float(c), stage1_delay_steps=M))
+        return tuple(out)
+
+    @property
+    def fine_to_coarse(self) -> np.ndarray:
+        coarse = np.asarray(self.coarse_dms, dtype=np.float64)
+        return np.argmin(np.abs(self.dm_trials[:, None] - coarse[None, :]), axis=1)
 
     @property
     def nu_bot_GHz(self) -> float:
@@ -136,6 +219,8 @@ class NodeConfig:
         ``dm_endpoint``)."""
         if self.dm_ranges:
             return np.concatenate([r.dm_trials for r in self.range_configs()])
+        if self.dm_list:
+            return np.asarray(self.dm_list, dtype=np.float64)
         return np.linspace(float(self.dm_min), float(self.dm_max), int(self.n_dm), endpoint=self.dm_endpoint)
 
     def range_configs(self) -> Tuple["NodeConfig", ...]:
@@ -202,6 +287,20 @@ class NodeConfig:
             if n_sat < self.min_stat_count:
                 raise ValueError(f"stats_tau_s={self.stats_tau_s}: the decayed sample weight saturates at "
                                  f"{n_sat:.0f} < min_stat_count={self.min_stat_count}; no DM would be scored")
+        if self.dm_list and len(self.dm_list) != self.n_dm:
+            raise ValueError(f"dm_list holds {len(self.dm_list)} trials, n_dm={self.n_dm}")
+        if self.stage1_dm is not None or self.coarse_dms:
+            if self.stage1_csf < 1 or self.n_chan % self.stage1_csf or (self.n_chan // self.stage1_csf) % self.n_sub:
+                raise ValueError(f"stage1_csf={self.stage1_csf} must divide the channels into whole groups "
+                                 f"per subband")
+            if self.stage1_csf > self.csf and self.stage1_csf % self.csf:
+                raise ValueError("a stage-1 channel group must hold whole correlator output channels")
+            if self.dm_ranges:
+                raise ValueError("stage 1 and an uneven DM plan do not combine")
+        if self.stage1_dm is not None and self.stage1_delay > self.n_steps:
+            raise ValueError(f"stage-1 delay {self.stage1_delay} steps exceeds the block ({self.n_steps} steps)")
+        if self.static_sky_blocks < 0 or self.static_sky_warmup < 0:
+            raise ValueError("static_sky_blocks and static_sky_warmup must be >= 0")
         for lo, hi, n, f, _ in self.dm_ranges:
             if int(f) < 1 or self.n_steps % (DETECT_CHUNK_STEPS * int(f)) != 0:
                 raise ValueError(f"DM range {lo}-{hi}: downsampling {f} must leave a multiple of "
@@ -243,6 +342,17 @@ def uneven_dm_config() -> NodeConfig:
     return search_node_config().with_(
         name="96ant_384ch_uneven", widths=(1, 2, 4, 8, 16, 32, 64), n_dm=288, dm_max=1500.0,
         dm_ranges=((0.0, 300.0, 128, 1, False), (300.0, 800.0, 96, 2, False), (800.0, 1500.0, 64, 4, True)))
+
+
+def coarse_dm_config() -> NodeConfig:
+    """The 96-antenna 384-channel node with stage 1: four coarse DMs (187.5, 562.5,
+    937.5, 1312.5) remove the delay inside each subband per 8-channel group
+    (the reference's 48 summed channels of a chgroup, ``chan_sum_factor`` 8)
+    before gridding; each of the 256 fine trials 0-1500 is searched on its
+    nearest coarse DM's stream; static sky subtracted over 8 blocks."""
+    return search_node_config().with_(
+        name="96ant_384ch_stage1", coarse_dms=(187.5, 562.5, 937.5, 1312.5), stage1_csf=8,
+        static_sky_blocks=8, static_sky_warmup=8)
 
 
 def search_node_short_config(n_samp: int = 1024) -> NodeConfig:

@@ -58,7 +58,7 @@ class SearchNode:
                  max_block_steps: Optional[int] = None, cand_cap: int = 1 << 20,
                  stream: Optional[torch.cuda.Stream] = None, lds_budget: Optional[int] = None,
                  dedisp_mode: Optional[int] = None, n_slots: int = 1, with_front: bool = True,
-                 with_detect: bool = True):
+                 with_detect: bool = True, share_front: Optional["SearchNode"] = None):
         cfg.validate()
         if ring_dtype not in (torch.float16, torch.float32):
             raise TypeError("ring_dtype must be float16 or float32")
@@ -101,10 +101,19 @@ class SearchNode:
             raise ValueError("n_slots must be >= 1")
         self.n_slots = int(n_slots)
         nf = self.n_slots if with_front else 0
-        self.vis_slots = [torch.empty((self.cap, cfg.n_chan_out, nb, 2), dtype=torch.int32, device=self.dev)
-                          for _ in range(nf)]
-        self.img_slots = [torch.empty(cfg.n_sub * self.cap * N * N, dtype=ring_dtype, device=self.dev)
-                          for _ in range(nf)]
+        if share_front is not None:
+            # nodes driven in turn on one stream (one per coarse DM) share the block buffers:
+            # stream order puts each node's reads of a slot before the next node's writes
+            if (share_front.cap < self.cap or share_front.n_slots < nf or share_front.ring_dtype != ring_dtype
+                    or tuple(share_front.vis_slots[0].shape[1:]) != (cfg.n_chan_out, nb, 2)):
+                raise ValueError("share_front: the other node's block buffers do not fit this node")
+            self.vis_slots = share_front.vis_slots[:nf]
+            self.img_slots = share_front.img_slots[:nf]
+        else:
+            self.vis_slots = [torch.empty((self.cap, cfg.n_chan_out, nb, 2), dtype=torch.int32, device=self.dev)
+                              for _ in range(nf)]
+            self.img_slots = [torch.empty(cfg.n_sub * self.cap * N * N, dtype=ring_dtype, device=self.dev)
+                              for _ in range(nf)]
         self.vis = self.vis_slots[0] if nf else None
         row_len = self.hist + self.cap
         self.row_len = row_len
@@ -155,7 +164,24 @@ class SearchNode:
         self.dedisp_mode = int(dedisp_mode)
         self.plans: Dict[int, tuple] = {}
         self.cal = None                                  # (gains re, im, time keep, scale) when calibrating
-        self.T0 = 0
+        # ---- stage 1: per-channel coarse-DM shifts on the correlator's operand load;
+        # the stage-1 stream trails the raw stream by s1_delay steps (its first block
+        # starts at step -s1_delay, the missing past read as zeros)
+        self.s1_delay = cfg.stage1_delay
+        self.s1_shift = None
+        self.s1_tail = None
+        if cfg.stage1_dm is not None and with_front:
+            self.s1_shift = torch.from_numpy(cfg.stage1_bins).to(self.dev)
+            n_tail = self.s1_delay * (cfg.t_int // 2) * cfg.n_ant * cfg.n_chan * 4
+            self.s1_tail = torch.zeros(max(n_tail, 4), dtype=torch.uint8, device=self.dev)
+        # ---- static sky: per-(subband, pixel) means of the last static_sky_blocks blocks
+        self.sky_ring = self.sky_sum = None
+        self.sky_head = self.sky_count = self.sky_seen = 0
+        if cfg.static_sky_blocks > 0 and with_front:
+            self.sky_ring = torch.zeros((cfg.static_sky_blocks, cfg.n_sub, self.n_pix), dtype=torch.float32,
+                                        device=self.dev)
+            self.sky_sum = torch.zeros((cfg.n_sub, self.n_pix), dtype=torch.float64, device=self.dev)
+        self.T0 = -self.s1_delay
         self.prev_len = 0
         self.marks: Optional[Dict[str, list]] = None     # stage -> [start, end] events when timing
 
@@ -242,7 +268,20 @@ class SearchNode:
             raise ValueError(f"{n_steps} steps > max_block_steps {self.cap}")
         vis = self.vis_slots[slot][:n_steps]
         self._mark("corr", 0)
-        if self.cal is None:
+        if self.s1_shift is not None:
+            if self.cal is not None:
+                raise ValueError("stage 1 and calibration do not combine in this node")
+            if n_steps < self.s1_delay:
+                raise ValueError(f"a {n_steps}-step block is shorter than the stage-1 delay ({self.s1_delay})")
+            s = self._stream(stream)
+            _lib.call("dsx_corr_i8_stage1", _lib.ptr(raw), _lib.ptr(self.s1_tail), _lib.ptr(self.s1_shift),
+                      self.s1_delay, _lib.ptr(vis), n_pkt, cfg.n_ant, cfg.n_chan, cfg.t_int, cfg.csf, 1, 1,
+                      _lib.stream(s))
+            n_tail = self.s1_delay * (cfg.t_int // 2) * cfg.n_ant * cfg.n_chan * 4
+            if n_tail:
+                with torch.cuda.stream(s):
+                    self.s1_tail[:n_tail].copy_(raw.reshape(-1)[raw.numel() - n_tail:])
+        elif self.cal is None:
             _lib.call("dsx_corr_i8", _lib.ptr(raw), None, _lib.ptr(vis), n_pkt, cfg.n_ant, cfg.n_chan,
                       cfg.t_int, cfg.csf, 1, 0, 1, _lib.stream(self._stream(stream)))
         else:
@@ -297,8 +336,25 @@ class SearchNode:
                   cfg.n_grid, self.n_rows, self.npad, 1.0,
                   _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
                   _lib.stream(self._stream(stream)))
+        if self.sky_ring is not None:
+            self._static_sky(img, n_steps, stream)
         self._mark("image", 1)
         return img
+
+    def _static_sky(self, img: torch.Tensor, n_steps: int, stream: Optional[torch.cuda.Stream]) -> None:
+        """Subtract the mean of the previous ``static_sky_blocks`` blocks' mean
+        images (after ``static_sky_warmup`` blocks) and push this block's mean
+        (``StaticSkyMean.apply``, corr_fast_integration.py:1049-1246, in the
+        image domain)."""
+        cfg = self.cfg
+        W = cfg.static_sky_blocks
+        subtract = int(self.sky_seen >= cfg.static_sky_warmup)
+        _lib.call("dsx_static_sky", _lib.ptr(img), _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
+                  cfg.n_sub, n_steps, self.n_pix, _lib.ptr(self.sky_ring), _lib.ptr(self.sky_sum), W, self.sky_head,
+                  self.sky_count, subtract, _lib.stream(self._stream(stream)))
+        self.sky_count = min(self.sky_count + 1, W)
+        self.sky_head = (self.sky_head + 1) % W
+        self.sky_seen += 1
 
     def push_images(self, img: torch.Tensor, n_steps: int, stream: Optional[torch.cuda.Stream] = None) -> None:
         cfg = self.cfg
@@ -576,6 +632,86 @@ class SearchNode:
         n_steps = self._check_block(n_samp)
         img = self.front(raw, n_samp)
         return self.finish(self.back(img, n_steps))
+
+
+class CoarseDMNode:
+    """Node stream with stage 1 (``cfg.coarse_dms``): per coarse DM, the
+    correlator reads every channel group ``stage1_bins`` steps later (its delay
+    inside the subband at that DM removed before gridding,
+    ``Stage1MultiDMCoarseDM``, corr_fast_integration.py:495-1040), images the
+    stage-1 stream (static sky subtracted when configured) into that coarse
+    DM's history, and searches the fine trials whose nearest coarse DM it is.
+    Per coarse DM this is a :class:`SearchNode`; the nodes share one set of
+    block buffers and run in turn on one stream.  Candidates of all groups are
+    decoded together (DM index offset by the earlier groups' trials); every
+    group's stream trails the raw stream by the same stage-1 delay, so block
+    times agree."""
+
+    def __init__(self, cfg: NodeConfig, antpos_e: np.ndarray, antpos_n: np.ndarray, *,
+                 device: str = "cuda", ring_dtype: torch.dtype = torch.float32,
+                 max_block_steps: Optional[int] = None, stream: Optional[torch.cuda.Stream] = None,
+                 cand_cap: int = 1 << 20):
+        if not cfg.coarse_dms:
+            raise ValueError("CoarseDMNode needs cfg.coarse_dms")
+        cfg.validate()
+        self.cfg = cfg
+        cap = int(max_block_steps or cfg.n_steps)
+        self.groups = []
+        first = None
+        off = 0
+        for gc in cfg.coarse_groups():
+            nd = SearchNode(gc, antpos_e, antpos_n, device=device, ring_dtype=ring_dtype, max_block_steps=cap,
+                            stream=stream, cand_cap=cand_cap, share_front=first)
+            first = first or nd
+            self.groups.append((nd, off))
+            off += gc.n_dm
+
+    @property
+    def stage1_delay(self) -> int:
+        return self.groups[0][0].s1_delay
+
+    def autotune(self, n_samp: Optional[int] = None) -> List[List[dict]]:
+        n_samp = self.cfg.n_samp if n_samp is None else int(n_samp)
+        n_steps = self.groups[0][0]._check_block(n_samp)
+        return [nd.autotune(n_steps) for nd, _ in self.groups]
+
+    def enable_timing(self) -> None:
+        for nd, _ in self.groups:
+            nd.enable_timing()
+
+    def stage_ms(self) -> Dict[str, float]:
+        out: Dict[str, float] = {}
+        for nd, _ in self.groups:
+            for k, v in nd.stage_ms().items():
+                out[k] = out.get(k, 0.0) + v
+        return out
+
+    def process_block(self, raw: torch.Tensor, n_samp: Optional[int] = None) -> BlockResult:
+        cfg = self.cfg
+        n_samp = cfg.n_samp if n_samp is None else int(n_samp)
+        pends = []
+        for nd, _ in self.groups:
+            n_steps = nd._check_block(n_samp)
+            img = nd.front(raw, n_samp)
+            pends.append(nd.back(img, n_steps))
+        recs = []
+        n_raw = 0
+        overflow = False
+        thr = []
+        for (nd, off), pend in zip(self.groups, pends):
+            res = nd.finish(pend)
+            n_raw += res.n_raw
+            overflow = overflow or res.overflow
+            thr.append(res.threshold_used)
+            r = res.records.copy()
+            r["d"] = r["d"] + off
+            recs.append(r)
+        rec = np.concatenate(recs) if recs else np.zeros(0, REC_DTYPE)
+        cands = decode_chunk_maxima(rec, widths=list(cfg.widths), n_grid=cfg.n_grid, dm_trials=cfg.dm_trials,
+                                    threshold=cfg.threshold, n_top=cfg.n_top, radius_t=cfg.radius_t,
+                                    radius_dm=cfg.radius_dm, radius_lm=cfg.radius_lm)
+        return BlockResult(candidates=cands, n_raw=int(n_raw), overflow=overflow, T0=int(pends[0].T0),
+                           n_steps=pends[0].n_steps, records=rec, threshold_used=max(thr), range_thresholds=thr)
 
 
 @dataclass

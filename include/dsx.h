@@ -73,6 +73,22 @@ int dsx_corr_i8(const void *raw, const void *im, int32_t *vis, int n_pkt, int n_
                 int n_chan, int t_int, int csf, int fuse_pol, int input_layout,
                 int out_layout, dsx_stream_t stream);
 
+/* Stage 1 of the coarse-DM plan folded into the correlator's operand load:
+ * output step t of the block reads channel c's samples of step
+ * t + chan_shift[c] - delay (device int32 [n_chan], each in [0, delay]);
+ * steps before the block come from `tail`, the previous block's last `delay`
+ * steps of raw bytes ((delay * t_int / 2, n_ant, n_chan, 2, 2), zeros before
+ * the stream).  The block's output covers steps [T0 - delay, T0 + L - delay)
+ * and stays exact int32.  The caller copies the block's last `delay` steps
+ * into the tail afterwards.  Other arguments as dsx_corr_i8 (raw input).
+ * Replaces Stage1MultiDMCoarseDM.dedisperse_from_vis (per-channel stage-1
+ * shifts on the visibilities before gridding, with the sliding 2-block
+ * window), src/dsart/services/corr_fast_integration.py:495-1040, with the
+ * shifts of DMPlan.delay_bins_per_subband, coarse_dm/dm_plan.py:479-515. */
+int dsx_corr_i8_stage1(const uint8_t *raw, const uint8_t *tail, const int32_t *chan_shift,
+                       int delay, int32_t *vis, int n_pkt, int n_ant, int n_chan, int t_int,
+                       int csf, int fuse_pol, int out_layout, dsx_stream_t stream);
+
 /* ----------------------------------------------- calibrated correlator
  * Complex gains (and the RFI mask, folded in as zero gains) applied to every
  * voltage as it is loaded; fp16 MFMA with float32 accumulation.
@@ -146,6 +162,19 @@ int dsx_image_subbands_f32(const float *vis, const int32_t *tap_src, const int32
                            const int32_t *tap_off, void *img, int n_steps, int n_sub,
                            int per_sub, int nbase, int n_grid, int n_rows, int npad,
                            float scale, int img_dtype, dsx_stream_t stream);
+
+/* Static-sky subtraction of a block of images img (n_sub, n_steps, n_pix)
+ * (img_dtype DSX_F32 / DSX_F16), per (subband, pixel): when subtract != 0 and
+ * count > 0 every step loses wsum / count (the mean of the previous `count`
+ * blocks' means); then this block's mean (float64 sum over its steps) enters
+ * the window: ring (window, n_sub, n_pix) float32 slot `head` (its old value
+ * leaves wsum when count == window) and wsum (n_sub, n_pix) float64.  The
+ * caller advances head and count.  Imaging is linear, so this is the image of
+ * the per-cell uv subtraction of StaticSkyMean.apply,
+ * src/dsart/services/corr_fast_integration.py:1049-1246. */
+int dsx_static_sky(void *img, int img_dtype, int n_sub, int n_steps, int64_t n_pix, float *ring,
+                   double *wsum, int window, int head, int count, int subtract,
+                   dsx_stream_t stream);
 
 /* Fused dequant + subband shift-and-add + C2R imager for a search cube.
  * For every trial f < n_fdm and cube time t in [t_lo, t_det):

@@ -111,6 +111,19 @@ def subband_images(vis_int: np.ndarray, rows: np.ndarray, cols: np.ndarray, n_gr
     return out
 
 
+def subband_grids(vis: np.ndarray, rows: np.ndarray, cols: np.ndarray, n_grid: int, n_sub: int) -> np.ndarray:
+    """Complex uv grids ``(n_sub, n_steps, N, N)`` of visibilities
+    ``(n_steps, n_chan_out, nbase, 2)`` (``np.add.at`` per subband)."""
+    n_steps, nco, nb, _ = vis.shape
+    per = nco // n_sub
+    v = vis[..., 0].astype(np.float64) + 1j * vis[..., 1].astype(np.float64)
+    out = np.zeros((n_sub, n_steps, n_grid, n_grid), dtype=np.complex128)
+    for s in range(n_sub):
+        sl = slice(s * per, (s + 1) * per)
+        out[s] = grid_plane(np.transpose(v[:, sl, :], (0, 2, 1)), rows[:, sl], cols[:, sl], n_grid)
+    return out
+
+
 def subband_images_sparse(vis_int: np.ndarray, rows: np.ndarray, cols: np.ndarray, n_grid: int,
                           n_sub: int, mask: np.ndarray, pix=None) -> np.ndarray:
     """Same images as :func:`subband_images`, ``(n_sub, n_steps, n_pix)``

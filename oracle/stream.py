@@ -22,6 +22,7 @@ import numpy as np
 from . import corr as ocorr
 from . import image as oimage
 from . import search as osearch
+from . import stage1 as os1
 
 DELAY = 64
 CHUNK = 64
@@ -38,6 +39,16 @@ class OracleStream:
             antpos_e, antpos_n, gf, cfg.n_grid, cfg.cell_lambda)
         per = nco // cfg.n_sub
         nu_ref = self.freq_out[::per][: cfg.n_sub]
+        self.s1_dm = getattr(cfg, "stage1_dm", None)
+        if self.s1_dm is not None:
+            # stage 1 aligns each subband to its first channel group
+            nu_ref = os1.subband_refs(f, n_sub=cfg.n_sub, group=cfg.stage1_csf)
+            self.s1_bins = os1.stage1_bins(f, n_sub=cfg.n_sub, group=cfg.stage1_csf, t_int=cfg.t_int,
+                                           dm=self.s1_dm)
+            self.s1_delay = max(int(getattr(cfg, "stage1_delay_steps", 0)), int(self.s1_bins.max()))
+            self.s1_tail = np.zeros((self.s1_delay * cfg.t_int // 2, cfg.n_ant, f.size, 2, 2), np.uint8)
+        W = int(getattr(cfg, "static_sky_blocks", 0))
+        self.sky = os1.StaticSkyMean(W, int(cfg.static_sky_warmup)) if W > 0 else None
         t_step = cfg.t_int * osearch.NATIVE_US
         # DM trial values are configuration (an explicit list when the config
         # carries one, e.g. one range of an uneven plan)
@@ -66,10 +77,30 @@ class OracleStream:
 
     def images(self, vis: np.ndarray) -> np.ndarray:
         """``(n_sub, n_steps, n_pix_sel)`` float64 (the sparse-product twin of
-        ``oracle.image.subband_images``, equal to it on integer visibilities)."""
+        ``oracle.image.subband_images``, equal to it on integer visibilities).
+        With static-sky subtraction the per-cell uv grids pass through
+        :class:`oracle.stage1.StaticSkyMean` before imaging."""
         c = self.cfg
-        return oimage.subband_images_sparse(vis, self.rows, self.cols, c.n_grid, c.n_sub, self.mask,
-                                            pix=self.pix)
+        if self.sky is None:
+            return oimage.subband_images_sparse(vis, self.rows, self.cols, c.n_grid, c.n_sub, self.mask,
+                                                pix=self.pix)
+        G = oimage.subband_grids(vis, self.rows, self.cols, c.n_grid, c.n_sub)     # (n_sub, steps, N, N)
+        G = np.transpose(self.sky.apply(np.transpose(G, (1, 0, 2, 3))), (1, 0, 2, 3))
+        img = oimage.image(G, self.mask)
+        return img.reshape(c.n_sub, G.shape[1], -1)[:, :, self.pix]
+
+    def stage1_visibilities(self, raw: np.ndarray, n_samp: int) -> np.ndarray:
+        """The stage-1 stream's visibilities of one raw block (steps
+        ``[T0 - delay, T0 + L - delay)``); keeps the block's last ``delay`` steps."""
+        c = self.cfg
+        nch = len(c.chan_freqs_GHz)
+        raw = np.asarray(raw, np.uint8).reshape(n_samp // 2, c.n_ant, nch, 2, 2)
+        v = os1.stage1_visibilities(self.s1_tail, raw, n_ant=c.n_ant, n_chan=nch, t_int=c.t_int, csf=c.csf,
+                                    bins=self.s1_bins, delay=self.s1_delay)
+        n_tail = self.s1_tail.shape[0]
+        if n_tail:
+            self.s1_tail = raw[raw.shape[0] - n_tail:].copy()
+        return v
 
     def start_at(self, t0: int) -> None:
         """Start the stream at step ``t0`` with zeros before it (as a node
@@ -110,7 +141,10 @@ class OracleStream:
         return cands
 
     def push_raw(self, raw: np.ndarray, n_samp: int):
-        vis = self.visibilities_fast(raw, n_samp)
+        if self.s1_dm is not None:
+            vis = self.stage1_visibilities(raw, n_samp)
+        else:
+            vis = self.visibilities_fast(raw, n_samp)
         return self.push_images(self.images(vis))
 
     def visibilities_fast(self, raw: np.ndarray, n_samp: int) -> np.ndarray:
