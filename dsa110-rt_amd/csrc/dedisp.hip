@@ -40,6 +40,7 @@
 #include <vector>
 
 #include "dsx_internal.h"
+#include "dedisp_plan.h"
 
 namespace dsx {
 
@@ -85,12 +86,6 @@ __device__ __forceinline__ float round_sig11(float x) {
     return __uint_as_float(u);
 }
 
-constexpr int DD_CHUNK = 64;      // centres per candidate chunk
-constexpr int DD_SUB = 32;        // centres per register pass (two per chunk)
-constexpr int DD_THREADS = 256;   // DMs per work-group (grid.y covers the rest)
-constexpr int DD_MAX_ROWS = 4096;
-constexpr int DD_STAGE_ITERS = 12;   // fast path: a subband window spans at most 12 x 64 steps
-constexpr int DD_STAGE_SUBS = 4;     // fast path: subbands a wave stages per batch of loads
 
 // Centred D'(t) = D(t) - bias of DM d from the term rows, in the main path's
 // order; t_rel = t - (first centre of the tile).
@@ -627,182 +622,9 @@ static int dispatch_nw(int nw, const DedispParams &P, int threads, size_t lds, d
     }
 }
 
-// LDS words per thread holding its term bases (matches the kernel's MAXT choice).
-static int tb_words(int n_terms) { return n_terms <= 8 ? 4 : n_terms <= 16 ? 8 : 0; }
-
-// Pair rows of a plan: per pair g the sorted distinct deltas and, per delta,
-// the range [lo, hi] of sh_a over the DMs that use it.
-struct PairRows {
-    std::vector<int> a, b, delta, lo, hi;   // one entry per row
-    std::vector<int> first;                  // first row of pair g, size n_pair + 1
-};
-
-static PairRows pair_rows(const int32_t *sh, int n_sub, int n_dm) {
-    PairRows pr;
-    const int n_pair = (n_sub + 1) / 2;
-    pr.first.push_back(0);
-    for (int g = 0; g < n_pair; ++g) {
-        const int a = 2 * g, b = 2 * g + 1 < n_sub ? 2 * g + 1 : -1;
-        std::vector<int> dl;
-        for (int d = 0; d < n_dm; ++d)
-            dl.push_back(b >= 0 ? sh[(int64_t)b * n_dm + d] - sh[(int64_t)a * n_dm + d] : 0);
-        std::vector<int> u = dl;
-        std::sort(u.begin(), u.end());
-        u.erase(std::unique(u.begin(), u.end()), u.end());
-        for (int v : u) {
-            int lo = INT32_MAX, hi = INT32_MIN;
-            for (int d = 0; d < n_dm; ++d)
-                if (dl[d] == v) {
-                    lo = std::min(lo, (int)sh[(int64_t)a * n_dm + d]);
-                    hi = std::max(hi, (int)sh[(int64_t)a * n_dm + d]);
-                }
-            pr.a.push_back(a);
-            pr.b.push_back(b);
-            pr.delta.push_back(v);
-            pr.lo.push_back(lo);
-            pr.hi.push_back(hi);
-        }
-        pr.first.push_back((int)pr.a.size());
-    }
-    return pr;
-}
-
 }  // namespace dsx
 
 using namespace dsx;
-
-extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, int n_widths,
-                               int n_centres, int ring_dtype, int lds_budget, int mode,
-                               int32_t *sub_info, int32_t *rows, int rows_cap, int32_t *tbase,
-                               int32_t *info) {
-    DSX_REQUIRE(host_shift && sub_info && rows && tbase && info, "dsx_dedisp_plan: null pointer");
-    DSX_REQUIRE(n_sub > 0 && n_sub <= 1024, "dsx_dedisp_plan: n_sub=%d not in [1, 1024]", n_sub);
-    DSX_REQUIRE(n_dm > 0 && n_dm <= 32767, "dsx_dedisp_plan: n_dm=%d", n_dm);
-    DSX_REQUIRE(n_widths >= 1 && n_widths <= 7, "dsx_dedisp_plan: n_widths=%d", n_widths);
-    DSX_REQUIRE(n_centres > 0 && n_centres % DD_CHUNK == 0,
-                "dsx_dedisp_plan: n_centres must be a positive multiple of 64");
-    DSX_REQUIRE(ring_dtype == DSX_F16 || ring_dtype == DSX_F32, "dsx_dedisp_plan: ring_dtype");
-    DSX_REQUIRE(mode >= 0 && mode <= 3, "dsx_dedisp_plan: mode=%d not in {0, 1, 2, 3}", mode);
-    DSX_REQUIRE(mode != 3 || ring_dtype == DSX_F32,
-                "dsx_dedisp_plan: mode 3 (LDS-direct window loads) needs a float32 history");
-    const int ww = n_widths > 1 ? (1 << (n_widths - 1)) : 0;
-    const int h2 = ww / 2;
-    std::vector<int> shmax(n_sub), shmin(n_sub);
-    for (int s = 0; s < n_sub; ++s) {
-        int hi = host_shift[(int64_t)s * n_dm], lo = hi;
-        for (int d = 1; d < n_dm; ++d) {
-            hi = std::max(hi, (int)host_shift[(int64_t)s * n_dm + d]);
-            lo = std::min(lo, (int)host_shift[(int64_t)s * n_dm + d]);
-        }
-        shmax[s] = hi;
-        shmin[s] = lo;
-    }
-    const PairRows pr = pair_rows(host_shift, n_sub, n_dm);
-    const int n_prow = (int)pr.a.size();
-    auto r_elems = [&](int tt) {
-        int64_t r = 0;
-        for (int s = 0; s < n_sub; ++s) r += tt + ww + shmax[s] - shmin[s];
-        return r;
-    };
-    auto q_elems = [&](int tt) {
-        int64_t q = 0;
-        for (int r = 0; r < n_prow; ++r) q += tt + ww + pr.hi[r] - pr.lo[r];
-        return q;
-    };
-    // largest tile within the LDS budget, preferring tiles whose windows fit
-    // the batched staging of the fast kernel
-    const int threads = std::min(DD_THREADS, ((n_dm + 63) / 64) * 64);
-    auto best_tile = [&](bool paired, bool dbl) {
-        int best = 0, best_fast = 0;
-        const int n_terms = paired ? (n_sub + 1) / 2 : n_sub;
-        for (int t = DD_CHUNK; t <= n_centres && t <= 2048; t += DD_CHUNK) {
-            const int64_t bytes = (paired ? q_elems(t) * 4 : 0) + r_elems(t) * 4 * (dbl ? 2 : 1) +
-                                  ((int64_t)tb_words(n_terms) * threads + 3 * n_sub) * 4;
-            if (n_centres % t != 0 || bytes > lds_budget) continue;
-            best = t;
-            bool fits = true;
-            for (int s2 = 0; s2 < n_sub; ++s2) fits = fits && t + ww + shmax[s2] - shmin[s2] <= 64 * DD_STAGE_ITERS;
-            if (fits) best_fast = t;
-        }
-        return best_fast > 0 ? best_fast : best;
-    };
-    bool paired = false;
-    int tt = 0;
-    if (mode != 2 && mode != 3 && n_prow <= DD_MAX_ROWS) {
-        tt = best_tile(true, false);
-        paired = tt > 0;
-    }
-    if (!paired) {
-        if (mode == 1)
-            return set_error(DSX_ERANGE, "dsx_dedisp_plan: pair rows need %lld B of LDS > %d",
-                             (long long)(q_elems(DD_CHUNK) * 4 + r_elems(DD_CHUNK) * 4), lds_budget);
-        tt = best_tile(false, mode == 3);
-    }
-    if (tt == 0)
-        return set_error(DSX_ERANGE, "dsx_dedisp_plan: the 64-centre tile needs %lld B of LDS > %d",
-                         (long long)(r_elems(DD_CHUNK) * 4), lds_budget);
-    const int n_rows = paired ? n_prow : 0;
-    if (n_rows > rows_cap)
-        return set_error(DSX_ERANGE, "dsx_dedisp_plan: %d pair rows > rows_cap %d", n_rows, rows_cap);
-    // subband windows: R index of step tau for subband s = off_s + tau - (tile start - W/2 - shmax_s)
-    int r_total = 0;
-    std::vector<int> roff(n_sub);
-    for (int s = 0; s < n_sub; ++s) {
-        const int len = tt + ww + shmax[s] - shmin[s];
-        roff[s] = r_total;
-        sub_info[3 * s] = r_total;
-        sub_info[3 * s + 1] = shmax[s];
-        sub_info[3 * s + 2] = len;
-        r_total += len;
-    }
-    int q_total = mode == 3 ? r_total : 0;     // mode 3: the second window buffer
-    if (paired) {
-        // row r covers tau in [tile start - W/2 - hi_r, + len_r) of its pair's DMs
-        std::vector<int> qoff(n_prow);
-        for (int r = 0; r < n_prow; ++r) {
-            const int a = pr.a[r], b = pr.b[r];
-            const int len = tt + ww + pr.hi[r] - pr.lo[r];
-            qoff[r] = q_total;
-            rows[6 * r] = q_total;
-            rows[6 * r + 1] = len;
-            rows[6 * r + 2] = roff[a] + shmax[a] - pr.hi[r];
-            if (b >= 0) {
-                rows[6 * r + 3] = roff[b] + shmax[b] - pr.hi[r] - pr.delta[r];
-                rows[6 * r + 4] = roff[b];
-                rows[6 * r + 5] = roff[b] + tt + ww + shmax[b] - shmin[b];
-            } else {
-                rows[6 * r + 3] = 0;
-                rows[6 * r + 4] = 0;
-                rows[6 * r + 5] = 0;
-            }
-            q_total += len;
-        }
-        const int n_pair = (n_sub + 1) / 2;
-        for (int g = 0; g < n_pair; ++g) {
-            const int a = 2 * g, b = 2 * g + 1 < n_sub ? 2 * g + 1 : -1;
-            for (int d = 0; d < n_dm; ++d) {
-                const int sa = host_shift[(int64_t)a * n_dm + d];
-                const int dl = b >= 0 ? host_shift[(int64_t)b * n_dm + d] - sa : 0;
-                int r = pr.first[g];
-                while (pr.delta[r] != dl) ++r;
-                tbase[(int64_t)g * n_dm + d] = qoff[r] + h2 + pr.hi[r] - sa;
-            }
-        }
-    } else {
-        for (int s = 0; s < n_sub; ++s)
-            for (int d = 0; d < n_dm; ++d)
-                tbase[(int64_t)s * n_dm + d] = roff[s] + h2 + shmax[s] - host_shift[(int64_t)s * n_dm + d];
-    }
-    const int n_terms = paired ? (n_sub + 1) / 2 : n_sub;
-    info[0] = tt;
-    info[1] = n_rows;
-    info[2] = n_terms;                              // terms per dedispersed sample
-    info[3] = r_total;
-    info[4] = q_total;
-    info[5] = paired ? 1 : 0;
-    info[6] = (int)(((int64_t)q_total + r_total + (int64_t)tb_words(n_terms) * threads + 3 * n_sub) * 4);
-    return DSX_OK;
-}
 
 extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, int n_pix, int hist,
                                  int ring_len, int64_t T0, const int32_t *sub_info, const int32_t *rows,

@@ -17,6 +17,9 @@ int set_error(int code, const char *fmt, ...);
 // Check the most recent launch; returns DSX_OK or DSX_ELAUNCH with the HIP error text.
 int check_launch(const char *what);
 
+#if defined(__HIPCC__)
+// device-side helpers (host-only translation units built by a plain C++
+// compiler, e.g. the planner under sanitizers, skip them)
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -25,6 +28,7 @@ __device__ __forceinline__ float ld_as_float(const _Float16 *p, int64_t i) { ret
 
 __device__ __forceinline__ void st_from_float(float *p, int64_t i, float v) { p[i] = v; }
 __device__ __forceinline__ void st_from_float(_Float16 *p, int64_t i, float v) { p[i] = (_Float16)v; }
+#endif
 
 inline hipStream_t as_stream(dsx_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -108,10 +108,20 @@ class SubbandImager:
 
 def dirty_image_from_uv_grid(uv_grid, *, out_dtype=None, prescale: float = 1.0):
     """``fftshift(Re(ifft2(prescale * uv_grid)))`` on the last two axes
-    (imager.py:248-301).  CUDA tensors run the HIP imager (``dsx_dirty_image``);
-    leading axes pass through.  ``out_dtype`` defaults to float32."""
+    (imager.py:248-301), on the HIP imager (``dsx_dirty_image``); leading axes
+    pass through.  A numpy grid is copied to the GPU and the image comes back
+    as numpy (float32 unless ``out_dtype``), as the reference's numpy path
+    returns numpy; a CUDA tensor stays on the device (float32 unless
+    ``out_dtype``)."""
     if not isinstance(uv_grid, torch.Tensor):
-        raise TypeError("dirty_image_from_uv_grid runs on the GPU; pass a CUDA tensor")
+        arr = np.asarray(uv_grid)
+        if arr.ndim < 2 or arr.shape[-1] != arr.shape[-2]:
+            raise ValueError(f"uv_grid must be [..., N, N]; got {arr.shape}")
+        if not torch.cuda.is_available():
+            raise RuntimeError("dirty_image_from_uv_grid runs on the GPU; no ROCm device is visible")
+        out = dirty_image_from_uv_grid(torch.from_numpy(np.ascontiguousarray(arr, dtype=np.complex64)).cuda(),
+                                       prescale=prescale).cpu().numpy()
+        return out if out_dtype is None else out.astype(out_dtype, copy=False)
     if uv_grid.dim() < 2 or uv_grid.shape[-1] != uv_grid.shape[-2]:
         raise ValueError(f"uv_grid must be [..., N, N]; got {tuple(uv_grid.shape)}")
     N = int(uv_grid.shape[-1])

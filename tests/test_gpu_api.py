@@ -105,6 +105,10 @@ def test_dirty_image_from_uv_grid(N, B):
     got = dirty_image_from_uv_grid(torch.from_numpy(G).cuda(), prescale=0.5).cpu().numpy()
     want = np.fft.fftshift(np.fft.ifft2(0.5 * G.astype(np.complex128), axes=(-2, -1)).real, axes=(-2, -1))
     assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+    # numpy in, numpy out (the reference's numpy path), same HIP imager
+    got_np = dirty_image_from_uv_grid(G, prescale=0.5)
+    assert isinstance(got_np, np.ndarray) and got_np.dtype == np.float32 and np.array_equal(got_np, got)
+    assert dirty_image_from_uv_grid(G, out_dtype=np.float64).dtype == np.float64
 
 
 def test_combine_chgroups_vs_numpy():
