@@ -463,7 +463,7 @@ def main():
     achieved = alg_bytes / (dd_ms * 1e-3) / 1e9
     plan = node.plans[cfg.n_steps][0]
     maxt = 8 if plan.n_terms <= 8 else 16 if plan.n_terms <= 16 else 0
-    is_async = plan.q_total > 0 and not plan.paired
+    is_async = plan.dma
     tname = "_Float16" if args.history == "fp16" else "float"
     kname = (f"dedisp_kernel<{tname}, {len(cfg.widths)}, {str(plan.paired).lower()}, {maxt}"
              + (", true>" if is_async else ">"))

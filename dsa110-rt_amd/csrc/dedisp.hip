@@ -144,6 +144,7 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
     float *Q = lds;                                   // [q_total] float32 pair rows (ASYNC: window buffer 1)
     float *R = lds + P.q_total;                       // [r_total] subband windows (ASYNC: window buffer 0)
     const float *X = PAIRED ? Q : R;                  // rows the lanes sum (ASYNC: this tile's buffer)
+    const float *Rw = R;                              // windows the pair rows are built from
     const T *ring = reinterpret_cast<const T *>(P.ring);
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
@@ -154,10 +155,11 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
     // term bases of this lane, two 16-bit LDS indices per word, in LDS slots
     // after the windows (kept out of registers: the chunk loop needs them all)
     constexpr int TBW = MAXT > 0 ? (MAXT + 1) / 2 : 1;
-    uint32_t *tb = reinterpret_cast<uint32_t *>(lds + P.q_total + P.r_total) + tid;
+    constexpr int RBUFS = (PAIRED && ASYNC) ? 2 : 1;   // pair rows with LDS-direct loads: two window buffers
+    uint32_t *tb = reinterpret_cast<uint32_t *>(lds + P.q_total + RBUFS * P.r_total) + tid;
     // subband window table (offset, shmax, length) copied to LDS, so the staging
     // loop's table reads never wait behind the window loads in flight
-    int *si = reinterpret_cast<int *>(lds + P.q_total + P.r_total) + (MAXT > 0 ? TBW * nthr : 0);
+    int *si = reinterpret_cast<int *>(lds + P.q_total + RBUFS * P.r_total) + (MAXT > 0 ? TBW * nthr : 0);
     for (int i = tid; i < 3 * P.n_sub; i += nthr) si[i] = P.sub_info[i];
     if constexpr (MAXT > 0) {
 #pragma unroll
@@ -178,12 +180,13 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
     bool fast = MAXT > 0 && P.n_sub <= n_waves * DD_STAGE_SUBS;
     for (int s = 0; s < P.n_sub; ++s) fast = fast && si[3 * s + 2] <= 64 * DD_STAGE_ITERS;
 
-    // ASYNC (float32 history, unpaired plan with two window buffers): the
-    // windows of the next tile -- same pixel, or the first tile of the
-    // work-group's next pixel -- are copied from HBM into the other buffer by
+    // ASYNC (float32 history): the windows of the next tile -- same pixel, or
+    // the first tile of the work-group's next pixel -- are copied from HBM by
     // LDS-direct loads (global_load_lds_dword: lane l fills slot base + l, no
-    // registers hold the data) while the current tile computes.  Steps outside
-    // the history are zero-filled by ordinary stores.
+    // registers hold the data) while the current tile computes, into the
+    // other of two window buffers (with pair rows: the buffer the next tile's
+    // rows are built from).  Steps outside the history are zero-filled by
+    // ordinary stores.
     auto dma_stage = [&](int pp, int cc0, float *dst) {
         const int64_t pos0 = (int64_t)P.hist + (P.c_lo + cc0) - H2 - P.T0;
         const float *ring_p = reinterpret_cast<const float *>(P.ring) + (int64_t)pp * P.row_len;
@@ -247,13 +250,20 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                 // then for every wave's (and for every wave to leave the other buffer)
                 __builtin_amdgcn_s_waitcnt(0);
                 __syncthreads();
-                X = (tile_no & 1) ? Q : R;
                 int np = p, nc = c0 + P.tt;
                 if (nc >= P.n_centres) {
                     np = p + (int)gridDim.x;
                     nc = 0;
                 }
-                if (np < P.n_pix) dma_stage(np, nc, (tile_no & 1) ? R : Q);
+                if constexpr (!PAIRED) {
+                    X = (tile_no & 1) ? Q : R;
+                    if (np < P.n_pix) dma_stage(np, nc, (tile_no & 1) ? R : Q);
+                } else {
+                    // pair rows: this tile's rows are built from one window buffer
+                    // while the next tile's windows stream into the other
+                    Rw = (tile_no & 1) ? R + P.r_total : R;
+                    if (np < P.n_pix) dma_stage(np, nc, (tile_no & 1) ? R : R + P.r_total);
+                }
                 ++tile_no;
             } else {
             __syncthreads();
@@ -328,8 +338,8 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                     for (int e = lane; e < len; e += 64) {
                         const int ib = b0 + e;
                         const bool okb = ib >= blo && ib < bhi;
-                        const float va = R[a0 + e];
-                        const float vb = R[okb ? ib : a0 + e];
+                        const float va = Rw[a0 + e];
+                        const float vb = Rw[okb ? ib : a0 + e];
                         Q[qoff + e] = okb ? va + vb : va;
                     }
                 }
@@ -580,23 +590,25 @@ static int launch_maxt(const DedispParams &P, int threads, size_t lds, dim3 grid
     return check_launch("dsx_dedisp_detect");
 }
 
-// float32 history, unpaired plan with two window buffers
-template <int NW>
+// float32 history with LDS-direct window loads: unpaired plan with two window
+// buffers, or pair rows with one window buffer
+template <int NW, bool PAIRED>
 static int launch_async(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
-    if (P.n_terms <= 8) return launch_maxt<float, NW, false, 8, true>(P, threads, lds, grid, s);
-    if (P.n_terms <= 16) return launch_maxt<float, NW, false, 16, true>(P, threads, lds, grid, s);
-    return launch_maxt<float, NW, false, 0, true>(P, threads, lds, grid, s);
+    if (P.n_terms <= 8) return launch_maxt<float, NW, PAIRED, 8, true>(P, threads, lds, grid, s);
+    if (P.n_terms <= 16) return launch_maxt<float, NW, PAIRED, 16, true>(P, threads, lds, grid, s);
+    return launch_maxt<float, NW, PAIRED, 0, true>(P, threads, lds, grid, s);
 }
 
+template <bool PAIRED>
 static int dispatch_async(int nw, const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
     switch (nw) {
-        case 1: return launch_async<1>(P, threads, lds, grid, s);
-        case 2: return launch_async<2>(P, threads, lds, grid, s);
-        case 3: return launch_async<3>(P, threads, lds, grid, s);
-        case 4: return launch_async<4>(P, threads, lds, grid, s);
-        case 5: return launch_async<5>(P, threads, lds, grid, s);
-        case 6: return launch_async<6>(P, threads, lds, grid, s);
-        case 7: return launch_async<7>(P, threads, lds, grid, s);
+        case 1: return launch_async<1, PAIRED>(P, threads, lds, grid, s);
+        case 2: return launch_async<2, PAIRED>(P, threads, lds, grid, s);
+        case 3: return launch_async<3, PAIRED>(P, threads, lds, grid, s);
+        case 4: return launch_async<4, PAIRED>(P, threads, lds, grid, s);
+        case 5: return launch_async<5, PAIRED>(P, threads, lds, grid, s);
+        case 6: return launch_async<6, PAIRED>(P, threads, lds, grid, s);
+        case 7: return launch_async<7, PAIRED>(P, threads, lds, grid, s);
         default: return set_error(DSX_ENOTSUP, "dsx_dedisp_detect: n_widths=%d not in [1, 7]", nw);
     }
 }
@@ -631,7 +643,7 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
                                  int n_rows, const int32_t *tbase, int n_terms, int n_dm,
                                  const int32_t *valid_from, const double *n_prev, const double *blk_count,
                                  double *mean, double *m2, int merge, int64_t c_lo, int n_centres,
-                                 int n_widths, int tile, int r_total, int q_total, float threshold,
+                                 int n_widths, int tile, int r_total, int q_total, int dma, float threshold,
                                  double min_count, void *cand, int cand_cap, int32_t *n_cand,
                                  dsx_stream_t stream) {
     DSX_REQUIRE(ring && sub_info && tbase && valid_from && n_prev && blk_count && mean && m2 && cand && n_cand,
@@ -641,11 +653,12 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
                 "dsx_dedisp_detect: bad shape");
     DSX_REQUIRE(n_dm <= 32767, "dsx_dedisp_detect: n_dm=%d > 32767", n_dm);
     const bool paired = n_rows > 0;
-    const bool async = !paired && q_total > 0;
+    const bool async = dma != 0;
     DSX_REQUIRE(paired ? (rows && n_terms == (n_sub + 1) / 2 && q_total > 0 && n_rows <= DD_MAX_ROWS)
-                       : (n_terms == n_sub && (q_total == 0 || (q_total == r_total && ring_dtype == DSX_F32))),
-                "dsx_dedisp_detect: n_rows=%d / n_terms=%d / q_total=%d do not describe a plan for n_sub=%d",
-                n_rows, n_terms, q_total, n_sub);
+                       : (n_terms == n_sub && (async ? q_total == r_total : q_total == 0)),
+                "dsx_dedisp_detect: n_rows=%d / n_terms=%d / q_total=%d / dma=%d do not describe a plan for n_sub=%d",
+                n_rows, n_terms, q_total, dma, n_sub);
+    DSX_REQUIRE(!async || ring_dtype == DSX_F32, "dsx_dedisp_detect: LDS-direct window loads need a float32 history");
     DSX_REQUIRE(n_centres > 0 && n_centres % DD_CHUNK == 0,
                 "dsx_dedisp_detect: n_centres must be a positive multiple of 64");
     DSX_REQUIRE(tile > 0 && tile % DD_CHUNK == 0 && n_centres % tile == 0,
@@ -653,8 +666,10 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
     DSX_REQUIRE(n_widths >= 1 && n_widths <= 7, "dsx_dedisp_detect: n_widths=%d", n_widths);
     DSX_REQUIRE(cand_cap >= 0 && r_total > 0, "dsx_dedisp_detect: sizes");
     const int threads = std::min(DD_THREADS, ((n_dm + 63) / 64) * 64);
+    // pair rows with LDS-direct loads keep two window buffers after the rows
+    const size_t r_bufs = (paired && async) ? 2 : 1;
     const size_t lds =
-        ((size_t)q_total + (size_t)r_total + (size_t)tb_words(n_terms) * threads + 3 * (size_t)n_sub) * 4;
+        ((size_t)q_total + r_bufs * (size_t)r_total + (size_t)tb_words(n_terms) * threads + 3 * (size_t)n_sub) * 4;
     if (lds > 160 * 1024)
         return set_error(DSX_ERANGE, "dsx_dedisp_detect: %zu B of LDS for the time windows", lds);
     DedispParams P{};
@@ -688,7 +703,8 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
     P.min_count = min_count;
     const dim3 grid((unsigned)std::min(n_pix, 1 << 20), (unsigned)((n_dm + DD_THREADS - 1) / DD_THREADS));
     hipStream_t s = as_stream(stream);
-    if (async) return dispatch_async(n_widths, P, threads, lds, grid, s);
+    if (async) return paired ? dispatch_async<true>(n_widths, P, threads, lds, grid, s)
+                             : dispatch_async<false>(n_widths, P, threads, lds, grid, s);
     if (ring_dtype == DSX_F16)
         return paired ? dispatch_nw<_Float16, true>(n_widths, P, threads, lds, grid, s)
                       : dispatch_nw<_Float16, false>(n_widths, P, threads, lds, grid, s);

@@ -15,6 +15,7 @@ namespace dsx {
 // the range [lo, hi] of sh_a over the DMs that use it.
 struct PairRows {
     std::vector<int> a, b, delta, lo, hi;   // one entry per row
+    std::vector<int> dmin;                   // first DM (lane order) using the row
     std::vector<int> first;                  // first row of pair g, size n_pair + 1
 };
 
@@ -31,12 +32,14 @@ static PairRows pair_rows(const int32_t *sh, int n_sub, int n_dm) {
         std::sort(u.begin(), u.end());
         u.erase(std::unique(u.begin(), u.end()), u.end());
         for (int v : u) {
-            int lo = INT32_MAX, hi = INT32_MIN;
+            int lo = INT32_MAX, hi = INT32_MIN, dm0 = n_dm;
             for (int d = 0; d < n_dm; ++d)
                 if (dl[d] == v) {
                     lo = std::min(lo, (int)sh[(int64_t)a * n_dm + d]);
                     hi = std::max(hi, (int)sh[(int64_t)a * n_dm + d]);
+                    dm0 = std::min(dm0, d);
                 }
+            pr.dmin.push_back(dm0);
             pr.a.push_back(a);
             pr.b.push_back(b);
             pr.delta.push_back(v);
@@ -62,9 +65,9 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
     DSX_REQUIRE(n_centres > 0 && n_centres % DD_CHUNK == 0,
                 "dsx_dedisp_plan: n_centres must be a positive multiple of 64");
     DSX_REQUIRE(ring_dtype == DSX_F16 || ring_dtype == DSX_F32, "dsx_dedisp_plan: ring_dtype");
-    DSX_REQUIRE(mode >= 0 && mode <= 3, "dsx_dedisp_plan: mode=%d not in {0, 1, 2, 3}", mode);
-    DSX_REQUIRE(mode != 3 || ring_dtype == DSX_F32,
-                "dsx_dedisp_plan: mode 3 (LDS-direct window loads) needs a float32 history");
+    DSX_REQUIRE(mode >= 0 && mode <= 4, "dsx_dedisp_plan: mode=%d not in {0, 1, 2, 3, 4}", mode);
+    DSX_REQUIRE((mode != 3 && mode != 4) || ring_dtype == DSX_F32,
+                "dsx_dedisp_plan: mode %d (LDS-direct window loads) needs a float32 history", mode);
     const int ww = n_widths > 1 ? (1 << (n_widths - 1)) : 0;
     const int h2 = ww / 2;
     std::vector<int> shmax(n_sub), shmin(n_sub);
@@ -84,11 +87,39 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
         for (int s = 0; s < n_sub; ++s) r += tt + ww + shmax[s] - shmin[s];
         return r;
     };
-    auto q_elems = [&](int tt) {
+    // pair-row offsets.  Lane d reads row r at qoff[r] + hi_r - sh_a[d] (+ the
+    // step), a run that descends with d inside a row.  With `banks`, each
+    // pair's rows are laid out in the order their DMs appear and every row is
+    // padded so that its first lane's address sits one bank below the last
+    // lane of the previous row: the lanes of a wave then read one descending
+    // run of distinct banks (64 of them, 4 bytes each) across row boundaries.
+    const bool banks = mode == 4;
+    auto q_layout = [&](int tt, std::vector<int> *qoff) {
         int64_t q = 0;
-        for (int r = 0; r < n_prow; ++r) q += tt + ww + pr.hi[r] - pr.lo[r];
+        std::vector<int> order(n_prow);
+        for (int r = 0; r < n_prow; ++r) order[r] = r;
+        if (banks)
+            for (size_t g = 0; g + 1 < pr.first.size(); ++g)
+                std::sort(order.begin() + pr.first[g], order.begin() + pr.first[g + 1],
+                          [&](int x, int y) { return pr.dmin[x] < pr.dmin[y]; });
+        if (qoff) qoff->assign(n_prow, 0);
+        int64_t prev_base = 0;     // address of the previous row's last lane
+        for (size_t g = 0; g + 1 < pr.first.size(); ++g)
+            for (int i = pr.first[g]; i < pr.first[g + 1]; ++i) {
+                const int r = order[i];
+                const int span = pr.hi[r] - pr.lo[r];
+                if (banks && i > pr.first[g]) {
+                    // want (q + span) = prev_base - 1  (mod 64)
+                    const int64_t want = ((prev_base - 1 - span) % 64 + 64) % 64;
+                    q += ((want - q) % 64 + 64) % 64;
+                }
+                if (qoff) (*qoff)[r] = (int)q;
+                prev_base = q;
+                q += tt + ww + span;
+            }
         return q;
     };
+    auto q_elems = [&](int tt) { return q_layout(tt, nullptr); };
     // largest tile within the LDS budget, preferring tiles whose windows fit
     // the batched staging of the fast kernel
     const int threads = std::min(DD_THREADS, ((n_dm + 63) / 64) * 64);
@@ -109,11 +140,11 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
     bool paired = false;
     int tt = 0;
     if (mode != 2 && mode != 3 && n_prow <= DD_MAX_ROWS) {
-        tt = best_tile(true, false);
+        tt = best_tile(true, mode == 4);
         paired = tt > 0;
     }
     if (!paired) {
-        if (mode == 1)
+        if (mode == 1 || mode == 4)
             return set_error(DSX_ERANGE, "dsx_dedisp_plan: pair rows need %lld B of LDS > %d",
                              (long long)(q_elems(DD_CHUNK) * 4 + r_elems(DD_CHUNK) * 4), lds_budget);
         tt = best_tile(false, mode == 3);
@@ -138,12 +169,12 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
     int q_total = mode == 3 ? r_total : 0;     // mode 3: the second window buffer
     if (paired) {
         // row r covers tau in [tile start - W/2 - hi_r, + len_r) of its pair's DMs
-        std::vector<int> qoff(n_prow);
+        std::vector<int> qoff;
+        q_total = (int)q_layout(tt, &qoff);
         for (int r = 0; r < n_prow; ++r) {
             const int a = pr.a[r], b = pr.b[r];
             const int len = tt + ww + pr.hi[r] - pr.lo[r];
-            qoff[r] = q_total;
-            rows[6 * r] = q_total;
+            rows[6 * r] = qoff[r];
             rows[6 * r + 1] = len;
             rows[6 * r + 2] = roff[a] + shmax[a] - pr.hi[r];
             if (b >= 0) {
@@ -155,7 +186,6 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
                 rows[6 * r + 4] = 0;
                 rows[6 * r + 5] = 0;
             }
-            q_total += len;
         }
         const int n_pair = (n_sub + 1) / 2;
         for (int g = 0; g < n_pair; ++g) {
@@ -180,7 +210,9 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
     info[3] = r_total;
     info[4] = q_total;
     info[5] = paired ? 1 : 0;
-    info[6] = (int)(((int64_t)q_total + r_total + (int64_t)tb_words(n_terms) * threads + 3 * n_sub) * 4);
+    const int64_t r_bufs = mode == 4 ? 2 : 1;     // mode 4: two window buffers after the pair rows
+    info[6] = (int)(((int64_t)q_total + r_bufs * r_total + (int64_t)tb_words(n_terms) * threads + 3 * n_sub) * 4);
+    info[7] = (mode == 3 || mode == 4) ? 1 : 0;   // windows by LDS-direct loads
     return DSX_OK;
 }
 

@@ -41,7 +41,7 @@ PROTOTYPES = {
     "dsx_downsample_time": [_P, _P, _I, _I, _I64, _I, _I, _P],
     "dsx_dedisp_plan": [_P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _P],
     "dsx_dedisp_detect": [_P, _I, _I, _I, _I, _I, _I64, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P,
-                          _I, _I64, _I, _I, _I, _I, _I, _F, _D, _P, _I, _P, _P],
+                          _I, _I64, _I, _I, _I, _I, _I, _I, _F, _D, _P, _I, _P, _P],
     "dsx_dedisp_prime": [_P, _I, _I, _I, _I, _I, _I64, _P, _I, _P, _P, _I64, _I, _P, _P],
     "dsx_stats_decay": [_P, _I64, _P, _I, _D, _P],
     "dsx_combine": [_P, _P, _P, _I, _I, _I64, _I, _I, _P],

@@ -198,6 +198,7 @@ class DedispPlan:
     sub_info: np.ndarray      # [n_sub, 3] int32
     rows: np.ndarray          # [n_rows, 6] int32
     tbase: np.ndarray         # [n_terms, n_dm] int32
+    dma: bool = False         # windows copied HBM -> LDS by LDS-direct loads during compute
 
 
 def plan_dedisp(shift_sd: np.ndarray, *, n_widths: int, n_centres: int, ring_dtype: torch.dtype,
@@ -206,7 +207,10 @@ def plan_dedisp(shift_sd: np.ndarray, *, n_widths: int, n_centres: int, ring_dty
     ``mode`` 0 = pair rows when they fit, 1 = pair rows only, 2 = unpaired,
     3 = unpaired with two window buffers, the next tile's windows loaded from
     HBM straight into LDS while the current tile computes (float32 history;
-    ``q_total`` = ``r_total`` marks the second buffer)."""
+    ``q_total`` = ``r_total`` marks the second buffer), 4 = pair rows whose
+    offsets spread a wave's 64 lanes over the LDS banks, built from one of
+    two window buffers while the next tile's windows load into the other
+    (float32 history)."""
     sh = np.ascontiguousarray(shift_sd, dtype=np.int32)
     if sh.ndim != 2:
         raise ValueError("shift_sd must be [n_sub, n_dm]")
@@ -217,15 +221,15 @@ def plan_dedisp(shift_sd: np.ndarray, *, n_widths: int, n_centres: int, ring_dty
     sub_info = np.zeros((n_sub, 3), np.int32)
     rows = np.zeros((max(rows_cap, 1), 6), np.int32)
     tbase = np.zeros((max(n_sub, n_pair), n_dm), np.int32)
-    info = np.zeros(7, np.int32)
+    info = np.zeros(8, np.int32)
     P = _lib._P
     _lib.call("dsx_dedisp_plan", sh.ctypes.data_as(P), n_sub, n_dm, n_widths, n_centres, code, int(lds_budget),
               int(mode), sub_info.ctypes.data_as(P), rows.ctypes.data_as(P), rows_cap, tbase.ctypes.data_as(P),
               info.ctypes.data_as(P))
-    tile, n_rows, n_terms, r_total, q_total, paired, lds = (int(x) for x in info)
+    tile, n_rows, n_terms, r_total, q_total, paired, lds, dma = (int(x) for x in info)
     return DedispPlan(tile=tile, n_rows=n_rows, n_terms=n_terms, r_total=r_total, q_total=q_total,
                       paired=bool(paired), lds_bytes=lds, sub_info=sub_info, rows=rows[:n_rows].copy(),
-                      tbase=tbase[:n_terms].copy())
+                      tbase=tbase[:n_terms].copy(), dma=bool(dma))
 
 
 def combine_chgroups(*, per_chgroup_streams: Mapping[int, torch.Tensor], time_shift_per_chgroup,

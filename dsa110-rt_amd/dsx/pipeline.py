@@ -438,7 +438,7 @@ class SearchNode:
                   _lib.ptr(pd["tbase"]), plan.n_terms, cfg.n_dm, _lib.ptr(self.valid_from),
                   _lib.ptr(n_prev), _lib.ptr(self.blk_count), _lib.ptr(mean), _lib.ptr(m2), int(bool(merge)),
                   T0 - DETECT_DELAY_STEPS, n_steps, self.n_widths, plan.tile, plan.r_total, plan.q_total,
-                  float(threshold), float(cfg.min_stat_count), _lib.ptr(cand),
+                  int(plan.dma), float(threshold), float(cfg.min_stat_count), _lib.ptr(cand),
                   self.cand_cap if cand_cap is None else int(cand_cap), _lib.ptr(n_cand), _lib.stream(s))
 
     AUTOTUNE_BUDGETS = (16 << 10, 24 << 10, 32 << 10, 48 << 10, 64 << 10, 80 << 10, 112 << 10, 160 << 10)
@@ -448,7 +448,7 @@ class SearchNode:
         window buffers) the planner offers for blocks of ``n_steps`` over the
         autotuner's LDS budgets."""
         if modes is None:
-            modes = (2, 1, 3) if self.ring_dtype == torch.float32 else (2, 1)
+            modes = (2, 1, 3, 4) if self.ring_dtype == torch.float32 else (2, 1)
         kw = dict(n_widths=self.n_widths, n_centres=n_steps, ring_dtype=self.ring_dtype)
         seen, out = set(), []
         for mode in modes:
@@ -457,7 +457,7 @@ class SearchNode:
                     plan = plan_dedisp(self.shifts_host.T, lds_budget=b, mode=mode, **kw)
                 except RuntimeError:
                     continue
-                key = (plan.tile, plan.paired, plan.q_total > 0 and not plan.paired)
+                key = (plan.tile, plan.paired, plan.dma)
                 if key not in seen:
                     seen.add(key)
                     out.append(plan)
@@ -477,7 +477,7 @@ class SearchNode:
         Runs on scratch copies of the statistics and candidate buffers, so the
         stream's state is untouched.  Returns the timings."""
         if modes is None:
-            modes = (2, 1, 3) if self.ring_dtype == torch.float32 else (2, 1)
+            modes = (2, 1, 3, 4) if self.ring_dtype == torch.float32 else (2, 1)
         s = self._stream(None)
         mean, m2 = torch.empty_like(self.mean), torch.empty_like(self.m2)
         cand = torch.empty_like(self.cand_slots[0])
@@ -490,7 +490,7 @@ class SearchNode:
                     plan = plan_dedisp(self.shifts_host.T, lds_budget=b, mode=mode, **kw)
                 except RuntimeError:
                     continue
-                key = (plan.tile, plan.paired, plan.q_total > 0 and not plan.paired)
+                key = (plan.tile, plan.paired, plan.dma)
                 if key in seen:
                     continue
                 seen.add(key)
@@ -510,7 +510,7 @@ class SearchNode:
                         e1.synchronize()
                         ms.append(e0.elapsed_time(e1))
                 t = min(ms)
-                rows.append({"tile": plan.tile, "paired": plan.paired, "async": bool(plan.q_total > 0 and not plan.paired),
+                rows.append({"tile": plan.tile, "paired": plan.paired, "async": bool(plan.dma),
                              "lds_bytes": plan.lds_bytes, "ms": t})
                 if best is None or t < best[0]:
                     best = (t, plan, pd)

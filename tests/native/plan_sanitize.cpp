@@ -21,7 +21,7 @@ static std::vector<int32_t> table(int n_sub, int n_dm, double span, unsigned see
 
 static int run(const std::vector<int32_t> &sh, int n_sub, int n_dm, int n_widths, int n_centres, int dtype,
                int budget, int mode) {
-    std::vector<int32_t> sub_info(3 * (size_t)n_sub), tbase((size_t)n_sub * n_dm), info(7);
+    std::vector<int32_t> sub_info(3 * (size_t)n_sub), tbase((size_t)n_sub * n_dm), info(8);
     const int rows_cap = n_sub * n_dm + 1;
     std::vector<int32_t> rows(6 * (size_t)rows_cap);
     const int rc = dsx_dedisp_plan(sh.data(), n_sub, n_dm, n_widths, n_centres, dtype, budget, mode,
@@ -45,7 +45,7 @@ int main() {
     for (const auto &sp : shapes) {
         for (double span : {0.0, 40.0, 160.0}) {
             const auto sh = table(sp[0], sp[1], span, 7u + (unsigned)sp[0]);
-            for (int mode = 0; mode <= 3; ++mode)
+            for (int mode = 0; mode <= 4; ++mode)
                 for (int dtype : {DSX_F32, DSX_F16})
                     for (int budget : {16 * 1024, 48 * 1024, 160 * 1024}) {
                         const int r = run(sh, sp[0], sp[1], sp[2], sp[3], dtype, budget, mode);
@@ -56,7 +56,7 @@ int main() {
         }
     }
     // argument errors are refused, never read out of bounds
-    std::vector<int32_t> sh(16 * 256, 0), si(48), rows(6), tb(16 * 256), info(7);
+    std::vector<int32_t> sh(16 * 256, 0), si(48), rows(6), tb(16 * 256), info(8);
     if (dsx_dedisp_plan(sh.data(), 16, 256, 6, 100, DSX_F32, 65536, 0, si.data(), rows.data(), 1, tb.data(),
                         info.data()) == DSX_OK) ++fails;
     if (dsx_dedisp_plan(sh.data(), 16, 256, 9, 2048, DSX_F32, 65536, 0, si.data(), rows.data(), 1, tb.data(),

@@ -44,10 +44,10 @@ def test_version_and_error_paths():
     p = ctypes.cast(buf, ctypes.c_void_p)
     rc = L.dsx_corr_i8(p, None, p, 4, 2, 1, 3, 1, 1, 0, 1, None)     # odd t_int
     assert rc == -1 and b"t_int" in L.dsx_last_error()
-    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, p, p, 1, 0, 65, 1, 64, 10, 0,
+    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, p, p, 1, 0, 65, 1, 64, 10, 0, 0,
                              ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
     assert rc == -1 and b"n_centres" in L.dsx_last_error()
-    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, None, p, 1, 0, 64, 1, 64, 10, 0,
+    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, None, p, 1, 0, 64, 1, 64, 10, 0, 0,
                              ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
     assert rc == -1 and b"null" in L.dsx_last_error()
     # the prime kernel's centres must be held by the ring
@@ -57,8 +57,12 @@ def test_version_and_error_paths():
     assert rc == -1 and b"power of two" in L.dsx_last_error()
     rc = L.dsx_ring_advance(p, p, p, 1, 1, 1, 1, 0, 1, 1, None)
     assert rc == -1 and b"differ" in L.dsx_last_error()
-    # two window buffers (q_total == r_total on an unpaired plan) need a float32 history (ring_dtype 1 = fp16)
-    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, p, p, 1, 0, 64, 1, 64, 10, 10,
+    # two window buffers loaded LDS-direct (dma, q_total == r_total on an unpaired plan) need a float32
+    # history (ring_dtype 1 = fp16); an unpaired plan with a second buffer but no dma is not a plan
+    rc = L.dsx_dedisp_detect(p, 1, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, p, p, 1, 0, 64, 1, 64, 10, 10, 1,
+                             ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
+    assert rc == -1 and b"float32 history" in L.dsx_last_error()
+    rc = L.dsx_dedisp_detect(p, 0, 1, 1, 0, 10, 0, p, None, 0, p, 1, 1, p, p, p, p, p, 1, 0, 64, 1, 64, 10, 10, 0,
                              ctypes.c_float(8.0), ctypes.c_double(1.0), p, 1, p, None)
     assert rc == -1 and b"describe a plan" in L.dsx_last_error()
     buf2 = ctypes.create_string_buffer(64)

@@ -51,7 +51,7 @@ def _model(plan, sh_sd, n_widths, tt, rng):
 
 @pytest.mark.parametrize("name,mode,dtype", [("96ant", 0, torch.float16), ("96ant", 2, torch.float16),
                                               ("96ant", 1, torch.float32), ("96ant", 3, torch.float32),
-                                              ("16ant", 0, torch.float16)])
+                                              ("96ant", 4, torch.float32), ("16ant", 0, torch.float16)])
 def test_plan_reproduces_direct_dedispersion(name, mode, dtype):
     cfg = dcfg.search_node_config() if name == "96ant" else dcfg.small_node_config()
     sh_sd = np.ascontiguousarray(cfg.shift_table().T)
@@ -67,8 +67,45 @@ def test_plan_reproduces_direct_dedispersion(name, mode, dtype):
     if mode == 3:     # two window buffers of the same layout; the second sits where pair rows would
         assert not plan.paired and plan.n_terms == cfg.n_sub and plan.q_total == plan.r_total
         assert plan.lds_bytes >= 2 * 4 * plan.r_total
+    assert plan.dma == (mode in (3, 4))
+    if mode == 4:
+        assert plan.paired and plan.n_terms == (cfg.n_sub + 1) // 2
     D, want = _model(plan, sh_sd, nw, plan.tile, np.random.default_rng(3))
     np.testing.assert_allclose(D, want, rtol=1e-12, atol=1e-12)
+
+
+def _bank_conflicts(plan, n_dm, t_span):
+    """Extra LDS cycles of the lanes' term reads: per term, wave of 64 DMs
+    and step, (largest number of distinct addresses that share one of 64
+    4-byte banks) - 1."""
+    extra = 0
+    for g in range(plan.n_terms):
+        for w0 in range(0, n_dm, 64):
+            base = plan.tbase[g][w0:w0 + 64]
+            for t in range(t_span):
+                addr = np.unique(base + t)
+                extra += np.bincount(addr % 64).max() - 1
+    return extra
+
+
+def test_mode4_rows_spread_a_wave_over_the_banks():
+    """The node's plan: with mode 4's row offsets the 64 DM lanes of a wave
+    reading one pair's rows at one step rarely share a bank at different
+    addresses (rounding makes a DM's delta jitter between neighbouring rows,
+    so one wave can touch rows whose lanes span more than 64 addresses);
+    the same rows packed back to back (mode 1) conflict on half the reads."""
+    cfg = dcfg.search_node_config()
+    sh_sd = cfg.shift_table().T
+    kw = dict(n_widths=6, n_centres=2048, ring_dtype=torch.float32, lds_budget=160 * 1024)
+    p4 = plan_dedisp(sh_sd, mode=4, **kw)
+    p1 = plan_dedisp(sh_sd, mode=1, **kw)
+    assert p4.paired and p1.paired and p4.n_rows == p1.n_rows == 96
+    base_reads = p4.n_terms * (cfg.n_dm // 64) * 64
+    c4, c1 = _bank_conflicts(p4, cfg.n_dm, 64), _bank_conflicts(p1, cfg.n_dm, 64)
+    print(f"extra bank cycles per read: mode 4 {c4 / base_reads:.3f}, mode 1 {c1 / base_reads:.3f}")
+    assert c4 <= 0.15 * base_reads and c4 * 3 <= c1
+    # the padding costs at most 63 floats per row
+    assert p4.q_total - p1.q_total <= 63 * p4.n_rows
 
 
 def test_plan_96ant_is_paired_with_12_deltas():
@@ -81,7 +118,7 @@ def test_plan_96ant_is_paired_with_12_deltas():
 def test_plan_odd_subbands_and_signed_shifts():
     rng = np.random.default_rng(11)
     sh = np.sort(rng.integers(-20, 40, size=(5, 37)), axis=1).astype(np.int32)
-    for mode in (1, 2):
+    for mode in (1, 2, 4):
         plan = plan_dedisp(sh, n_widths=4, n_centres=256, ring_dtype=torch.float32, lds_budget=160 * 1024,
                            mode=mode)
         D, want = _model(plan, sh, 4, plan.tile, rng)

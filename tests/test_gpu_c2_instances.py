@@ -49,10 +49,10 @@ def test_node_plan_every_layout_vs_oracle(ring_dtype):
     probe = SearchNode(cfg, e, n, ring_dtype=ring_dtype)
     plans = probe.layouts(L)
     del probe
-    kinds = {(p.paired, p.q_total > 0 and not p.paired) for p in plans}
+    kinds = {(p.paired, p.dma) for p in plans}
     if ring_dtype == torch.float32:
-        assert kinds == {(False, False), (True, False), (False, True)}, kinds
-        assert {64, 128, 256} <= {p.tile for p in plans if p.q_total > 0 and not p.paired}
+        assert kinds == {(False, False), (True, False), (False, True), (True, True)}, kinds
+        assert {64, 128, 256} <= {p.tile for p in plans if p.dma and not p.paired}
     else:
         assert kinds == {(False, False), (True, False)}, kinds
     nodes = []
@@ -60,6 +60,7 @@ def test_node_plan_every_layout_vs_oracle(ring_dtype):
         nd = SearchNode(cfg, e, n, ring_dtype=ring_dtype)
         nd.use_plan(L, p)
         nodes.append((f"tile {p.tile} {'pairs' if p.paired else 'two-buffer' if p.q_total else 'one-buffer'}"
+                      f"{' lds-direct' if p.dma else ''}"
                       f" terms {p.n_terms}", nd))
     print(f"{ring_dtype}: {len(nodes)} layouts: {[k for k, _ in nodes]}", flush=True)
     o = OracleStream(cfg, e, n, pix=np.arange(N * N))
