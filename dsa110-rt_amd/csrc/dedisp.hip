@@ -87,6 +87,17 @@ __device__ __forceinline__ float round_sig11(float x) {
 }
 
 
+// BF16 storage of subband-pair rows (the reduced-precision partial sums): the
+// float32 pair sum rounded to nearest even, read back by a 16-bit shift.
+struct bf16s {
+    uint16_t u;
+    __device__ __forceinline__ operator float() const { return __uint_as_float((uint32_t)u << 16); }
+};
+__device__ __forceinline__ bf16s to_bf16(float f) {
+    const uint32_t x = __float_as_uint(f);
+    return bf16s{(uint16_t)((x + 0x7FFFu + ((x >> 16) & 1u)) >> 16)};
+}
+
 // Centred D'(t) = D(t) - bias of DM d from the term rows, in the main path's
 // order; t_rel = t - (first centre of the tile).
 template <typename E>
@@ -134,8 +145,10 @@ __device__ __forceinline__ void sum_terms(float (&Pw)[NP], const E *X, const uin
     }
 }
 
-template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false>
+template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false, bool QB = false>
 __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
+    static_assert(!QB || (PAIRED && !ASYNC), "BF16 rows are a layout of the pair-row plan");
+    using XE = std::conditional_t<QB, bf16s, float>;  // element of the rows the lanes sum
     constexpr int WMAX = 1 << (NW - 1);
     constexpr int H2 = WMAX / 2;
     constexpr int WW = 2 * H2;
@@ -143,7 +156,8 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
     extern __shared__ float lds[];
     float *Q = lds;                                   // [q_total] float32 pair rows (ASYNC: window buffer 1)
     float *R = lds + P.q_total;                       // [r_total] subband windows (ASYNC: window buffer 0)
-    const float *X = PAIRED ? Q : R;                  // rows the lanes sum (ASYNC: this tile's buffer)
+    XE *Qx = reinterpret_cast<XE *>(lds);              // pair rows in their storage type
+    const XE *X = PAIRED ? Qx : reinterpret_cast<const XE *>(R);   // rows the lanes sum (ASYNC: this tile's buffer)
     const float *Rw = R;                              // windows the pair rows are built from
     const T *ring = reinterpret_cast<const T *>(P.ring);
     const int tid = threadIdx.x;
@@ -340,7 +354,9 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                         const bool okb = ib >= blo && ib < bhi;
                         const float va = Rw[a0 + e];
                         const float vb = Rw[okb ? ib : a0 + e];
-                        Q[qoff + e] = okb ? va + vb : va;
+                        const float v = okb ? va + vb : va;
+                        if constexpr (QB) Qx[qoff + e] = to_bf16(v);
+                        else Q[qoff + e] = v;
                     }
                 }
             }
@@ -582,11 +598,11 @@ __global__ __launch_bounds__(DD_THREADS) void dedisp_prime_kernel(const T *ring,
 }
 
 // Plans with at most 8 (or 16) terms keep the lanes' term bases in registers.
-template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false>
+template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false, bool QB = false>
 static int launch_maxt(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
-    (void)hipFuncSetAttribute((const void *)dedisp_kernel<T, NW, PAIRED, MAXT, ASYNC>,
+    (void)hipFuncSetAttribute((const void *)dedisp_kernel<T, NW, PAIRED, MAXT, ASYNC, QB>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    dedisp_kernel<T, NW, PAIRED, MAXT, ASYNC><<<grid, threads, lds, s>>>(P);
+    dedisp_kernel<T, NW, PAIRED, MAXT, ASYNC, QB><<<grid, threads, lds, s>>>(P);
     return check_launch("dsx_dedisp_detect");
 }
 
@@ -613,23 +629,23 @@ static int dispatch_async(int nw, const DedispParams &P, int threads, size_t lds
     }
 }
 
-template <typename T, int NW, bool PAIRED>
+template <typename T, int NW, bool PAIRED, bool QB>
 static int launch_dedisp(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
-    if (P.n_terms <= 8) return launch_maxt<T, NW, PAIRED, 8>(P, threads, lds, grid, s);
-    if (P.n_terms <= 16) return launch_maxt<T, NW, PAIRED, 16>(P, threads, lds, grid, s);
-    return launch_maxt<T, NW, PAIRED, 0>(P, threads, lds, grid, s);
+    if (P.n_terms <= 8) return launch_maxt<T, NW, PAIRED, 8, false, QB>(P, threads, lds, grid, s);
+    if (P.n_terms <= 16) return launch_maxt<T, NW, PAIRED, 16, false, QB>(P, threads, lds, grid, s);
+    return launch_maxt<T, NW, PAIRED, 0, false, QB>(P, threads, lds, grid, s);
 }
 
-template <typename T, bool PAIRED>
+template <typename T, bool PAIRED, bool QB = false>
 static int dispatch_nw(int nw, const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
     switch (nw) {
-        case 1: return launch_dedisp<T, 1, PAIRED>(P, threads, lds, grid, s);
-        case 2: return launch_dedisp<T, 2, PAIRED>(P, threads, lds, grid, s);
-        case 3: return launch_dedisp<T, 3, PAIRED>(P, threads, lds, grid, s);
-        case 4: return launch_dedisp<T, 4, PAIRED>(P, threads, lds, grid, s);
-        case 5: return launch_dedisp<T, 5, PAIRED>(P, threads, lds, grid, s);
-        case 6: return launch_dedisp<T, 6, PAIRED>(P, threads, lds, grid, s);
-        case 7: return launch_dedisp<T, 7, PAIRED>(P, threads, lds, grid, s);
+        case 1: return launch_dedisp<T, 1, PAIRED, QB>(P, threads, lds, grid, s);
+        case 2: return launch_dedisp<T, 2, PAIRED, QB>(P, threads, lds, grid, s);
+        case 3: return launch_dedisp<T, 3, PAIRED, QB>(P, threads, lds, grid, s);
+        case 4: return launch_dedisp<T, 4, PAIRED, QB>(P, threads, lds, grid, s);
+        case 5: return launch_dedisp<T, 5, PAIRED, QB>(P, threads, lds, grid, s);
+        case 6: return launch_dedisp<T, 6, PAIRED, QB>(P, threads, lds, grid, s);
+        case 7: return launch_dedisp<T, 7, PAIRED, QB>(P, threads, lds, grid, s);
         default: return set_error(DSX_ENOTSUP, "dsx_dedisp_detect: n_widths=%d not in [1, 7]", nw);
     }
 }
@@ -643,7 +659,7 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
                                  int n_rows, const int32_t *tbase, int n_terms, int n_dm,
                                  const int32_t *valid_from, const double *n_prev, const double *blk_count,
                                  double *mean, double *m2, int merge, int64_t c_lo, int n_centres,
-                                 int n_widths, int tile, int r_total, int q_total, int dma, float threshold,
+                                 int n_widths, int tile, int r_total, int q_total, int flags, float threshold,
                                  double min_count, void *cand, int cand_cap, int32_t *n_cand,
                                  dsx_stream_t stream) {
     DSX_REQUIRE(ring && sub_info && tbase && valid_from && n_prev && blk_count && mean && m2 && cand && n_cand,
@@ -653,11 +669,14 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
                 "dsx_dedisp_detect: bad shape");
     DSX_REQUIRE(n_dm <= 32767, "dsx_dedisp_detect: n_dm=%d > 32767", n_dm);
     const bool paired = n_rows > 0;
-    const bool async = dma != 0;
+    const bool async = (flags & DSX_DD_DMA) != 0;
+    const bool row_bf16 = (flags & DSX_DD_ROWS_BF16) != 0;
+    DSX_REQUIRE((flags & ~(DSX_DD_DMA | DSX_DD_ROWS_BF16)) == 0, "dsx_dedisp_detect: flags=%d", flags);
     DSX_REQUIRE(paired ? (rows && n_terms == (n_sub + 1) / 2 && q_total > 0 && n_rows <= DD_MAX_ROWS)
-                       : (n_terms == n_sub && (async ? q_total == r_total : q_total == 0)),
-                "dsx_dedisp_detect: n_rows=%d / n_terms=%d / q_total=%d / dma=%d do not describe a plan for n_sub=%d",
-                n_rows, n_terms, q_total, dma, n_sub);
+                       : (n_terms == n_sub && !row_bf16 && (async ? q_total == r_total : q_total == 0)),
+                "dsx_dedisp_detect: n_rows=%d / n_terms=%d / q_total=%d / flags=%d do not describe a plan for n_sub=%d",
+                n_rows, n_terms, q_total, flags, n_sub);
+    DSX_REQUIRE(!(row_bf16 && async), "dsx_dedisp_detect: BF16 pair rows are not combined with LDS-direct loads");
     DSX_REQUIRE(!async || ring_dtype == DSX_F32, "dsx_dedisp_detect: LDS-direct window loads need a float32 history");
     DSX_REQUIRE(n_centres > 0 && n_centres % DD_CHUNK == 0,
                 "dsx_dedisp_detect: n_centres must be a positive multiple of 64");
@@ -705,6 +724,9 @@ extern "C" int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, in
     hipStream_t s = as_stream(stream);
     if (async) return paired ? dispatch_async<true>(n_widths, P, threads, lds, grid, s)
                              : dispatch_async<false>(n_widths, P, threads, lds, grid, s);
+    if (row_bf16)
+        return ring_dtype == DSX_F16 ? dispatch_nw<_Float16, true, true>(n_widths, P, threads, lds, grid, s)
+                                     : dispatch_nw<float, true, true>(n_widths, P, threads, lds, grid, s);
     if (ring_dtype == DSX_F16)
         return paired ? dispatch_nw<_Float16, true>(n_widths, P, threads, lds, grid, s)
                       : dispatch_nw<_Float16, false>(n_widths, P, threads, lds, grid, s);

@@ -65,7 +65,7 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
     DSX_REQUIRE(n_centres > 0 && n_centres % DD_CHUNK == 0,
                 "dsx_dedisp_plan: n_centres must be a positive multiple of 64");
     DSX_REQUIRE(ring_dtype == DSX_F16 || ring_dtype == DSX_F32, "dsx_dedisp_plan: ring_dtype");
-    DSX_REQUIRE(mode >= 0 && mode <= 4, "dsx_dedisp_plan: mode=%d not in {0, 1, 2, 3, 4}", mode);
+    DSX_REQUIRE(mode >= 0 && mode <= 5, "dsx_dedisp_plan: mode=%d not in [0, 5]", mode);
     DSX_REQUIRE((mode != 3 && mode != 4) || ring_dtype == DSX_F32,
                 "dsx_dedisp_plan: mode %d (LDS-direct window loads) needs a float32 history", mode);
     const int ww = n_widths > 1 ? (1 << (n_widths - 1)) : 0;
@@ -119,7 +119,9 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
             }
         return q;
     };
-    auto q_elems = [&](int tt) { return q_layout(tt, nullptr); };
+    // bytes of one pair-row element: mode 5 stores the rows as BF16
+    const int q_esz = mode == 5 ? 2 : 4;
+    auto q_elems = [&](int tt) { return (q_layout(tt, nullptr) * q_esz + 3) / 4; };   // in 4-byte words
     // largest tile within the LDS budget, preferring tiles whose windows fit
     // the batched staging of the fast kernel
     const int threads = std::min(DD_THREADS, ((n_dm + 63) / 64) * 64);
@@ -144,7 +146,7 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
         paired = tt > 0;
     }
     if (!paired) {
-        if (mode == 1 || mode == 4)
+        if (mode == 1 || mode == 4 || mode == 5)
             return set_error(DSX_ERANGE, "dsx_dedisp_plan: pair rows need %lld B of LDS > %d",
                              (long long)(q_elems(DD_CHUNK) * 4 + r_elems(DD_CHUNK) * 4), lds_budget);
         tt = best_tile(false, mode == 3);
@@ -169,8 +171,8 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
     int q_total = mode == 3 ? r_total : 0;     // mode 3: the second window buffer
     if (paired) {
         // row r covers tau in [tile start - W/2 - hi_r, + len_r) of its pair's DMs
-        std::vector<int> qoff;
-        q_total = (int)q_layout(tt, &qoff);
+        std::vector<int> qoff;                         // in row elements
+        q_total = (int)((q_layout(tt, &qoff) * q_esz + 3) / 4);
         for (int r = 0; r < n_prow; ++r) {
             const int a = pr.a[r], b = pr.b[r];
             const int len = tt + ww + pr.hi[r] - pr.lo[r];
@@ -212,7 +214,7 @@ extern "C" int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, i
     info[5] = paired ? 1 : 0;
     const int64_t r_bufs = mode == 4 ? 2 : 1;     // mode 4: two window buffers after the pair rows
     info[6] = (int)(((int64_t)q_total + r_bufs * r_total + (int64_t)tb_words(n_terms) * threads + 3 * n_sub) * 4);
-    info[7] = (mode == 3 || mode == 4) ? 1 : 0;   // windows by LDS-direct loads
+    info[7] = ((mode == 3 || mode == 4) ? DSX_DD_DMA : 0) | (mode == 5 ? DSX_DD_ROWS_BF16 : 0);
     return DSX_OK;
 }
 

@@ -199,6 +199,12 @@ class DedispPlan:
     rows: np.ndarray          # [n_rows, 6] int32
     tbase: np.ndarray         # [n_terms, n_dm] int32
     dma: bool = False         # windows copied HBM -> LDS by LDS-direct loads during compute
+    rows_bf16: bool = False   # subband-pair rows stored as BF16
+
+    @property
+    def flags(self) -> int:
+        """``DSX_DD_*`` layout flags of ``dsx_dedisp_detect``."""
+        return (1 if self.dma else 0) | (2 if self.rows_bf16 else 0)
 
 
 def plan_dedisp(shift_sd: np.ndarray, *, n_widths: int, n_centres: int, ring_dtype: torch.dtype,
@@ -210,7 +216,8 @@ def plan_dedisp(shift_sd: np.ndarray, *, n_widths: int, n_centres: int, ring_dty
     ``q_total`` = ``r_total`` marks the second buffer), 4 = pair rows whose
     offsets spread a wave's 64 lanes over the LDS banks, built from one of
     two window buffers while the next tile's windows load into the other
-    (float32 history)."""
+    (float32 history), 5 = pair rows as mode 1 stored as BF16 (the
+    reduced-precision partial sums)."""
     sh = np.ascontiguousarray(shift_sd, dtype=np.int32)
     if sh.ndim != 2:
         raise ValueError("shift_sd must be [n_sub, n_dm]")
@@ -226,10 +233,10 @@ def plan_dedisp(shift_sd: np.ndarray, *, n_widths: int, n_centres: int, ring_dty
     _lib.call("dsx_dedisp_plan", sh.ctypes.data_as(P), n_sub, n_dm, n_widths, n_centres, code, int(lds_budget),
               int(mode), sub_info.ctypes.data_as(P), rows.ctypes.data_as(P), rows_cap, tbase.ctypes.data_as(P),
               info.ctypes.data_as(P))
-    tile, n_rows, n_terms, r_total, q_total, paired, lds, dma = (int(x) for x in info)
+    tile, n_rows, n_terms, r_total, q_total, paired, lds, flags = (int(x) for x in info)
     return DedispPlan(tile=tile, n_rows=n_rows, n_terms=n_terms, r_total=r_total, q_total=q_total,
                       paired=bool(paired), lds_bytes=lds, sub_info=sub_info, rows=rows[:n_rows].copy(),
-                      tbase=tbase[:n_terms].copy(), dma=bool(dma))
+                      tbase=tbase[:n_terms].copy(), dma=bool(flags & 1), rows_bf16=bool(flags & 2))
 
 
 def combine_chgroups(*, per_chgroup_streams: Mapping[int, torch.Tensor], time_shift_per_chgroup,

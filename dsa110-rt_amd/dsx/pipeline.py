@@ -156,8 +156,8 @@ class SearchNode:
         # loads (tile 128 within 48 KB); fp16 history -> one unpaired buffer (80 KB)
         if dedisp_mode is None:
             dedisp_mode = 3 if ring_dtype == torch.float32 else 2
-        if dedisp_mode == 3 and ring_dtype != torch.float32:
-            raise ValueError("dedisp_mode 3 (LDS-direct window loads) needs a float32 history")
+        if dedisp_mode in (3, 4) and ring_dtype != torch.float32:
+            raise ValueError(f"dedisp_mode {dedisp_mode} (LDS-direct window loads) needs a float32 history")
         if lds_budget is None:
             lds_budget = LDS_BUDGET_ASYNC if dedisp_mode == 3 else LDS_BUDGET_DEDISP
         self.lds_budget = int(lds_budget)
@@ -438,7 +438,7 @@ class SearchNode:
                   _lib.ptr(pd["tbase"]), plan.n_terms, cfg.n_dm, _lib.ptr(self.valid_from),
                   _lib.ptr(n_prev), _lib.ptr(self.blk_count), _lib.ptr(mean), _lib.ptr(m2), int(bool(merge)),
                   T0 - DETECT_DELAY_STEPS, n_steps, self.n_widths, plan.tile, plan.r_total, plan.q_total,
-                  int(plan.dma), float(threshold), float(cfg.min_stat_count), _lib.ptr(cand),
+                  int(plan.flags), float(threshold), float(cfg.min_stat_count), _lib.ptr(cand),
                   self.cand_cap if cand_cap is None else int(cand_cap), _lib.ptr(n_cand), _lib.stream(s))
 
     AUTOTUNE_BUDGETS = (16 << 10, 24 << 10, 32 << 10, 48 << 10, 64 << 10, 80 << 10, 112 << 10, 160 << 10)
@@ -457,7 +457,7 @@ class SearchNode:
                     plan = plan_dedisp(self.shifts_host.T, lds_budget=b, mode=mode, **kw)
                 except RuntimeError:
                     continue
-                key = (plan.tile, plan.paired, plan.dma)
+                key = (plan.tile, plan.paired, plan.flags)
                 if key not in seen:
                     seen.add(key)
                     out.append(plan)
@@ -490,7 +490,7 @@ class SearchNode:
                     plan = plan_dedisp(self.shifts_host.T, lds_budget=b, mode=mode, **kw)
                 except RuntimeError:
                     continue
-                key = (plan.tile, plan.paired, plan.dma)
+                key = (plan.tile, plan.paired, plan.flags)
                 if key in seen:
                     continue
                 seen.add(key)

@@ -39,6 +39,12 @@ enum {
 
 enum { DSX_F32 = 0, DSX_F16 = 1, DSX_I8 = 2, DSX_I32 = 3 };
 
+/* dsx_dedisp_plan / dsx_dedisp_detect layout flags */
+enum {
+    DSX_DD_DMA = 1,        /* windows copied HBM -> LDS by LDS-direct loads during compute */
+    DSX_DD_ROWS_BF16 = 2,  /* subband-pair rows stored as BF16 */
+};
+
 /* Library version, (major << 16) | (minor << 8) | patch. */
 int dsx_version(void);
 
@@ -241,14 +247,15 @@ int dsx_downsample_time(const void *in, void *out, int n_planes, int n_in, int64
  * built from one of two window buffers while the next tile's windows stream
  * into the other by LDS-direct loads (float32 history only; LDS holds the
  * rows, then 2 x r_total), row offsets padded so that the 64 DM lanes of a
- * wave reading one pair's rows at one step spread over the 64 LDS banks.
- * Outputs (host memory):
+ * wave reading one pair's rows at one step spread over the 64 LDS banks;
+ * 5: pair rows as mode 1, stored as BF16 (the pair sums rounded to nearest
+ * even; the reduced-precision partial-sum variant).  Outputs (host memory):
  *   sub_info [n_sub][3] = (window offset, max_d shift[s][d], window length)
  *   rows     [rows_cap][6] pair rows (Q offset, length, R index of subband a
  *            at e = 0, R index of subband b at e = 0, b's window [lo, hi))
  *   tbase    [n_terms][n_dm] LDS index of each term of D(t) at t = tile start
- *   info[8]  = (tile, n_rows, n_terms, r_total, q_total, paired, LDS bytes per
- *              work-group, dma: 1 when the windows come by LDS-direct loads)
+ *   info[8]  = (tile, n_rows, n_terms, r_total, q_total (4-byte words), paired,
+ *              LDS bytes per work-group, DSX_DD_* flags)
  * LDS holds float32 subband windows and pair rows.  Pair rows: for subbands a = 2g, b = 2g + 1 and each distinct
  * delta = shift[b][d] - shift[a][d], Q[tau] = ring_a[tau] + ring_b[tau - delta]
  * over the span of steps the DMs with that delta read; D(t) is then the sum
@@ -280,9 +287,10 @@ int dsx_dedisp_plan(const int32_t *host_shift, int n_sub, int n_dm, int n_widths
  * mean rounded to 11 significant bits (see dsx_dedisp_prime for a DM's
  * first block), so the sums keep their precision for pixels that hold a
  * steady source.  sub_info, rows, tbase, n_rows, n_terms, tile, r_total,
- * q_total and dma come from dsx_dedisp_plan (copied to device memory; rows
- * may be NULL when n_rows == 0; dma != 0 selects the LDS-direct window loads
- * of modes 3 and 4 and needs ring_dtype DSX_F32).
+ * q_total and flags come from dsx_dedisp_plan (copied to device memory;
+ * rows may be NULL when n_rows == 0; DSX_DD_DMA selects the LDS-direct
+ * window loads of modes 3 and 4 and needs ring_dtype DSX_F32;
+ * DSX_DD_ROWS_BF16 the BF16 pair rows of mode 5).
  * ring_dtype DSX_F16 or DSX_F32; hist >= 64 + w_max/2 + max shift.
  * Replaces combine_chgroups + boxcar_via_cumsum + multi_boxcar_argmax +
  * the Layer-1/Layer-2 normalisation, src/dsart/fine_dm/combiner.py:548-655,
@@ -294,7 +302,7 @@ int dsx_dedisp_detect(const void *ring, int ring_dtype, int n_sub, int n_pix, in
                       int n_rows, const int32_t *tbase, int n_terms, int n_dm,
                       const int32_t *valid_from, const double *n_prev, const double *blk_count,
                       double *mean, double *m2, int merge, int64_t c_lo, int n_centres,
-                      int n_widths, int tile, int r_total, int q_total, int dma, float threshold,
+                      int n_widths, int tile, int r_total, int q_total, int flags, float threshold,
                       double min_count, void *cand, int cand_cap, int32_t *n_cand,
                       dsx_stream_t stream);
 

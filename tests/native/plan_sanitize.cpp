@@ -27,8 +27,10 @@ static int run(const std::vector<int32_t> &sh, int n_sub, int n_dm, int n_widths
     const int rc = dsx_dedisp_plan(sh.data(), n_sub, n_dm, n_widths, n_centres, dtype, budget, mode,
                                    sub_info.data(), rows.data(), rows_cap, tbase.data(), info.data());
     if (rc == DSX_OK) {
-        // every term base lies inside the windows / rows the plan reserves
-        const int64_t total = (int64_t)info[3] + info[4];
+        // every term base lies inside the windows / rows the plan reserves (pair rows
+        // are indexed in row elements: two per 4-byte word when stored as BF16)
+        const int64_t total = info[5] ? (int64_t)info[4] * ((info[7] & DSX_DD_ROWS_BF16) ? 2 : 1)
+                                      : (int64_t)info[3] + info[4];
         for (int64_t i = 0; i < (int64_t)info[2] * n_dm; ++i)
             if (tbase[i] < 0 || tbase[i] >= total) {
                 fprintf(stderr, "term base %d outside [0, %lld)\n", tbase[i], (long long)total);
@@ -45,7 +47,7 @@ int main() {
     for (const auto &sp : shapes) {
         for (double span : {0.0, 40.0, 160.0}) {
             const auto sh = table(sp[0], sp[1], span, 7u + (unsigned)sp[0]);
-            for (int mode = 0; mode <= 4; ++mode)
+            for (int mode = 0; mode <= 5; ++mode)
                 for (int dtype : {DSX_F32, DSX_F16})
                     for (int budget : {16 * 1024, 48 * 1024, 160 * 1024}) {
                         const int r = run(sh, sp[0], sp[1], sp[2], sp[3], dtype, budget, mode);

@@ -11,6 +11,13 @@ from dsx.dedisp import plan_dedisp
 from oracle import search as osearch
 
 
+def _bf16(x):
+    """float -> BF16 (round to nearest even) -> float."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
 def _model(plan, sh_sd, n_widths, tt, rng):
     """Stage random subband windows, build pair rows, sum the terms."""
     n_sub, n_dm = sh_sd.shape
@@ -27,14 +34,15 @@ def _model(plan, sh_sd, n_widths, tt, rng):
         tau = np.arange(ln) - h2 - smax
         R[off:off + ln] = series[s, tau - T0]
     if plan.paired:
-        X = np.zeros(plan.q_total)
+        n_el = plan.q_total * (2 if plan.rows_bf16 else 1)      # BF16 rows: two per 4-byte word
+        X = np.zeros(n_el)
         for qoff, ln, a0, b0, blo, bhi in plan.rows:
-            assert 0 <= a0 and a0 + ln <= plan.r_total and qoff + ln <= plan.q_total
+            assert 0 <= a0 and a0 + ln <= plan.r_total and qoff + ln <= n_el
             e = np.arange(ln)
             v = R[a0 + e].copy()
             ok = (b0 + e >= blo) & (b0 + e < bhi)
             v[ok] += R[(b0 + e)[ok]]
-            X[qoff:qoff + ln] = v
+            X[qoff:qoff + ln] = _bf16(v) if plan.rows_bf16 else v
     else:
         X = R
     t_rel = np.arange(-h2, tt + h2)
@@ -51,7 +59,8 @@ def _model(plan, sh_sd, n_widths, tt, rng):
 
 @pytest.mark.parametrize("name,mode,dtype", [("96ant", 0, torch.float16), ("96ant", 2, torch.float16),
                                               ("96ant", 1, torch.float32), ("96ant", 3, torch.float32),
-                                              ("96ant", 4, torch.float32), ("16ant", 0, torch.float16)])
+                                              ("96ant", 4, torch.float32), ("96ant", 5, torch.float16),
+                                              ("16ant", 0, torch.float16)])
 def test_plan_reproduces_direct_dedispersion(name, mode, dtype):
     cfg = dcfg.search_node_config() if name == "96ant" else dcfg.small_node_config()
     sh_sd = np.ascontiguousarray(cfg.shift_table().T)
@@ -67,11 +76,15 @@ def test_plan_reproduces_direct_dedispersion(name, mode, dtype):
     if mode == 3:     # two window buffers of the same layout; the second sits where pair rows would
         assert not plan.paired and plan.n_terms == cfg.n_sub and plan.q_total == plan.r_total
         assert plan.lds_bytes >= 2 * 4 * plan.r_total
-    assert plan.dma == (mode in (3, 4))
+    assert plan.dma == (mode in (3, 4)) and plan.rows_bf16 == (mode == 5)
     if mode == 4:
         assert plan.paired and plan.n_terms == (cfg.n_sub + 1) // 2
     D, want = _model(plan, sh_sd, nw, plan.tile, np.random.default_rng(3))
-    np.testing.assert_allclose(D, want, rtol=1e-12, atol=1e-12)
+    if plan.rows_bf16:      # each of the n_sub / 2 rows carries one BF16 rounding (2^-9 relative)
+        assert np.abs(D - want).max() <= plan.n_terms * 2.0 ** -8 * 4.0
+        assert np.abs(D - want).max() > 0
+    else:
+        np.testing.assert_allclose(D, want, rtol=1e-12, atol=1e-12)
 
 
 def _bank_conflicts(plan, n_dm, t_span):
