@@ -112,12 +112,41 @@ __device__ __forceinline__ float terms_d(const E *X, const int32_t *tbase, int n
 // summed as ((x_0 - bias) + x_1) + ...  With MAXT > 0 the term bases sit in
 // this lane's LDS slots (tb[h * stride], two 16-bit bases per word) and the
 // term loop is unrolled; otherwise they are read from global memory term by
-// term.
-template <int MAXT, int N, int OFF, typename E, int NP>
+// term.  EXACT: the plan has exactly MAXT terms (software-pipelined reads).
+template <int MAXT, int N, int OFF, bool EXACT, typename E, int NP>
 __device__ __forceinline__ void sum_terms(float (&Pw)[NP], const E *X, const uint32_t *tb, int stride,
                                           const int32_t *tbase, int n_terms, int n_dm, int d, int rel,
                                           float bias) {
-    if constexpr (MAXT > 0) {
+    if constexpr (EXACT) {
+        // exactly MAXT terms: term g + 1's N samples are read into one register
+        // buffer while term g's are added from the other, so a wave keeps N / 2
+        // LDS reads in flight behind its adds (the scheduling barriers keep the
+        // two phases apart).  Same summation order as the guarded loop.
+        uint32_t tw[(MAXT + 1) / 2];
+#pragma unroll
+        for (int h = 0; h < (MAXT + 1) / 2; ++h) tw[h] = tb[h * stride];
+        float buf[2][N];
+        {
+            const E *x0 = X + (int)(tw[0] & 0xFFFF) + rel;
+#pragma unroll
+            for (int j = 0; j < N; ++j) buf[0][j] = (float)x0[j];
+        }
+#pragma unroll
+        for (int g = 0; g < MAXT; ++g) {
+            if (g + 1 < MAXT) {
+                const E *xn = X + (int)((tw[(g + 1) >> 1] >> (16 * ((g + 1) & 1))) & 0xFFFF) + rel;
+#pragma unroll
+                for (int j = 0; j < N; ++j) buf[(g + 1) & 1][j] = (float)xn[j];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                if (g == 0) Pw[OFF + j] = buf[0][j] - bias;
+                else Pw[OFF + j] += buf[g & 1][j];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else if constexpr (MAXT > 0) {
         {
             const E *x0 = X + (int)(tb[0] & 0xFFFF) + rel;
 #pragma unroll
@@ -145,14 +174,18 @@ __device__ __forceinline__ void sum_terms(float (&Pw)[NP], const E *X, const uin
     }
 }
 
-template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false, bool QB = false>
+template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false, bool QB = false, bool EXACT = false>
 __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
     static_assert(!QB || (PAIRED && !ASYNC), "BF16 rows are a layout of the pair-row plan");
+    static_assert(!EXACT || MAXT > 0, "an exact term count is a register-base instance");
     using XE = std::conditional_t<QB, bf16s, float>;  // element of the rows the lanes sum
     constexpr int WMAX = 1 << (NW - 1);
     constexpr int H2 = WMAX / 2;
     constexpr int WW = 2 * H2;
-    constexpr int NP = DD_SUB + WW + 1;              // register prefix window
+    // centres per register pass: 16 for the pipelined exact-term sums (their
+    // second read buffer takes the registers of the smaller prefix window)
+    constexpr int SUB = EXACT ? DD_SUB / 2 : DD_SUB;
+    constexpr int NP = SUB + WW + 1;                 // register prefix window
     extern __shared__ float lds[];
     float *Q = lds;                                   // [q_total] float32 pair rows (ASYNC: window buffer 1)
     float *R = lds + P.q_total;                       // [r_total] subband windows (ASYNC: window buffer 0)
@@ -365,7 +398,7 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
 
             if (c0 == 0) {
                 // prologue: D(c_lo - H2 + x), x < WW, into Pw[x + 1]
-                if (WW > 0) sum_terms<MAXT, WW, 1>(Pw, X, tb, nthr, P.tbase, P.n_terms, P.n_dm, d, -H2, bias);
+                if (WW > 0) sum_terms<MAXT, WW, 1, false>(Pw, X, tb, nthr, P.tbase, P.n_terms, P.n_dm, d, -H2, bias);
                 // samples t = c_lo + (x - H2) for x >= H2 belong to this block
                 float a1 = 0.f, a2 = 0.f, m = -INFINITY;
 #pragma unroll
@@ -395,21 +428,21 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                 float m1n = -INFINITY;                           // width-1 maximum of the next chunk
                 mb[0] = m1_tail;
 #pragma unroll
-                for (int j = 0; j < DD_CHUNK / DD_SUB; ++j) {
-                    const int cs = cc + j * DD_SUB;                  // sub-chunk start (tile relative)
+                for (int j = 0; j < DD_CHUNK / SUB; ++j) {
+                    const int cs = cc + j * SUB;                  // sub-chunk start (tile relative)
                     // ---- D(c + j S + H2 + i), i < S, summed over the term rows
-                    sum_terms<MAXT, DD_SUB, WW + 1>(Pw, X, tb, nthr, P.tbase, P.n_terms, P.n_dm, d, cs + H2,
+                    sum_terms<MAXT, SUB, WW + 1, EXACT>(Pw, X, tb, nthr, P.tbase, P.n_terms, P.n_dm, d, cs + H2,
                                                     bias);
                     // ---- statistics of this block's complete samples
-                    const int64_t t1 = c + j * DD_SUB + H2;
+                    const int64_t t1 = c + j * SUB + H2;
                     float a1 = 0.f, a2 = 0.f;
-                    const bool full = t1 >= vf && t1 + DD_SUB <= c_hi;
+                    const bool full = t1 >= vf && t1 + SUB <= c_hi;
                     if (full) {
 #pragma unroll
-                        for (int i = 0; i < DD_SUB; ++i) a2 = fmaf(Pw[WW + 1 + i], Pw[WW + 1 + i], a2);
+                        for (int i = 0; i < SUB; ++i) a2 = fmaf(Pw[WW + 1 + i], Pw[WW + 1 + i], a2);
                     } else {
 #pragma unroll
-                        for (int i = 0; i < DD_SUB; ++i) {
+                        for (int i = 0; i < SUB; ++i) {
                             const float v = Pw[WW + 1 + i];
                             const bool ok = t1 + i >= vf && t1 + i < c_hi;
                             a1 += ok ? v : 0.f;
@@ -417,16 +450,16 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                         }
                     }
                     // ---- width-1 maxima from the fresh samples: t < c + 64 belong to this chunk
-                    const int split = DD_CHUNK - H2 - j * DD_SUB;
+                    const int split = DD_CHUNK - H2 - j * SUB;
 #pragma unroll
-                    for (int i = 0; i < DD_SUB; ++i) {
+                    for (int i = 0; i < SUB; ++i) {
                         if (i < split) mb[0] = fmaxf(mb[0], Pw[WW + 1 + i]);
                         else m1n = fmaxf(m1n, Pw[WW + 1 + i]);
                     }
                     // ---- prefix sums in place
 #pragma unroll
-                    for (int i = 0; i < DD_SUB; ++i) Pw[WW + 1 + i] += Pw[WW + i];
-                    if (full) a1 = Pw[WW + DD_SUB] - Pw[WW];
+                    for (int i = 0; i < SUB; ++i) Pw[WW + 1 + i] += Pw[WW + i];
+                    if (full) a1 = Pw[WW + SUB] - Pw[WW];
                     s1 += (double)a1;
                     s2 += (double)a2;
                     // ---- boxcar maxima of the wider widths over this sub-chunk's centres
@@ -437,13 +470,13 @@ __global__ __launch_bounds__(DD_THREADS, 2) void dedisp_kernel(DedispParams P) {
                             const int lo = H2 - (w >> 1);
                             float m = mb[k];
 #pragma unroll
-                            for (int i = 0; i < DD_SUB; ++i) m = fmaxf(m, Pw[i + lo + w] - Pw[i + lo]);
+                            for (int i = 0; i < SUB; ++i) m = fmaxf(m, Pw[i + lo + w] - Pw[i + lo]);
                             mb[k] = m;
                         }
                     }
                     // ---- slide the prefix window by one sub-chunk
 #pragma unroll
-                    for (int x = 0; x <= WW; ++x) Pw[x] = Pw[x + DD_SUB];
+                    for (int x = 0; x <= WW; ++x) Pw[x] = Pw[x + SUB];
                 }
                 m1_tail = m1n;
 
@@ -598,11 +631,11 @@ __global__ __launch_bounds__(DD_THREADS) void dedisp_prime_kernel(const T *ring,
 }
 
 // Plans with at most 8 (or 16) terms keep the lanes' term bases in registers.
-template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false, bool QB = false>
+template <typename T, int NW, bool PAIRED, int MAXT, bool ASYNC = false, bool QB = false, bool EXACT = false>
 static int launch_maxt(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
-    (void)hipFuncSetAttribute((const void *)dedisp_kernel<T, NW, PAIRED, MAXT, ASYNC, QB>,
+    (void)hipFuncSetAttribute((const void *)dedisp_kernel<T, NW, PAIRED, MAXT, ASYNC, QB, EXACT>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    dedisp_kernel<T, NW, PAIRED, MAXT, ASYNC, QB><<<grid, threads, lds, s>>>(P);
+    dedisp_kernel<T, NW, PAIRED, MAXT, ASYNC, QB, EXACT><<<grid, threads, lds, s>>>(P);
     return check_launch("dsx_dedisp_detect");
 }
 
@@ -610,6 +643,7 @@ static int launch_maxt(const DedispParams &P, int threads, size_t lds, dim3 grid
 // buffers, or pair rows with one window buffer
 template <int NW, bool PAIRED>
 static int launch_async(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
+    if (!PAIRED && P.n_terms == 16) return launch_maxt<float, NW, PAIRED, 16, true, false, true>(P, threads, lds, grid, s);
     if (P.n_terms <= 8) return launch_maxt<float, NW, PAIRED, 8, true>(P, threads, lds, grid, s);
     if (P.n_terms <= 16) return launch_maxt<float, NW, PAIRED, 16, true>(P, threads, lds, grid, s);
     return launch_maxt<float, NW, PAIRED, 0, true>(P, threads, lds, grid, s);
