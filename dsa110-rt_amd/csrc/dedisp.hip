@@ -643,7 +643,12 @@ static int launch_maxt(const DedispParams &P, int threads, size_t lds, dim3 grid
 // buffers, or pair rows with one window buffer
 template <int NW, bool PAIRED>
 static int launch_async(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
-    if (!PAIRED && P.n_terms == 16) return launch_maxt<float, NW, PAIRED, 16, true, false, true>(P, threads, lds, grid, s);
+    // exact term counts: the pipelined sums
+    if constexpr (!PAIRED) {
+        if (P.n_terms == 16) return launch_maxt<float, NW, PAIRED, 16, true, false, true>(P, threads, lds, grid, s);
+    } else {
+        if (P.n_terms == 8) return launch_maxt<float, NW, PAIRED, 8, true, false, true>(P, threads, lds, grid, s);
+    }
     if (P.n_terms <= 8) return launch_maxt<float, NW, PAIRED, 8, true>(P, threads, lds, grid, s);
     if (P.n_terms <= 16) return launch_maxt<float, NW, PAIRED, 16, true>(P, threads, lds, grid, s);
     return launch_maxt<float, NW, PAIRED, 0, true>(P, threads, lds, grid, s);
@@ -665,6 +670,11 @@ static int dispatch_async(int nw, const DedispParams &P, int threads, size_t lds
 
 template <typename T, int NW, bool PAIRED, bool QB>
 static int launch_dedisp(const DedispParams &P, int threads, size_t lds, dim3 grid, hipStream_t s) {
+    if constexpr (!QB && !PAIRED) {
+        if (P.n_terms == 16) return launch_maxt<T, NW, PAIRED, 16, false, false, true>(P, threads, lds, grid, s);
+    } else if constexpr (!QB) {
+        if (P.n_terms == 8) return launch_maxt<T, NW, PAIRED, 8, false, false, true>(P, threads, lds, grid, s);
+    }
     if (P.n_terms <= 8) return launch_maxt<T, NW, PAIRED, 8, false, QB>(P, threads, lds, grid, s);
     if (P.n_terms <= 16) return launch_maxt<T, NW, PAIRED, 16, false, QB>(P, threads, lds, grid, s);
     return launch_maxt<T, NW, PAIRED, 0, false, QB>(P, threads, lds, grid, s);
