@@ -29,7 +29,12 @@ import numpy as np
 import torch
 
 HBM_PEAK_GBPS = 8000.0
-LDS_PEAK_TBPS = 256 * 128 * 2.4e9 / 1e12     # MI355X LDS read bandwidth (128 B/clk/CU)
+# LDS read throughput of the whole GPU, measured by dsa110-rt_amd/csrc/probe/lds_read_probe.hip
+# (lane bytes per second, 256 CUs at the 2.1-2.3 GHz clock held under load): 16-byte reads of
+# consecutive words 135.6 TB/s (243 B/clk/CU); ds_read2_b32 at clustered per-lane addresses --
+# the dedispersion kernel's window reads, neighbouring DMs a few words apart -- 67.5 TB/s
+LDS_PEAK_TBPS = 135.6
+LDS_PATTERN_PEAK_TBPS = 67.5
 INT8_PEAK_POPS = 5.0                           # MI355X dense int8 MFMA
 MFMA_I8_PEAK_TOPS = 5000.0
 FP32_PEAK_TFLOPS = 157.3
@@ -493,19 +498,22 @@ def main():
                      "alg_bytes_per_launch": alg_bytes, "avg_ms": dd_ms,
                      "note": (f"algorithmic bytes = {args.history} subband images read once "
                               f"({ring_bytes}) + float64 mean/m2 read+write ({stat_bytes}); the "
-                              "kernel is not HBM-bound: per (DM, step, pixel) 16 float32 window reads from LDS "
-                              "and ~28 VALU instructions (hardware counters: 27.9 VALU and 8.7 LDS instructions "
-                              "per 64 lanes); with two window buffers filled by LDS-direct loads (fp32 history) "
-                              "VALU issues on 70% of SIMD cycles: bound by VALU issue and latency")},
+                              "kernel is not HBM-bound: per (DM, step, pixel) it reads 16 float32 window "
+                              "samples from LDS at per-lane addresses (64 lane bytes) and issues ~29 VALU "
+                              "instructions; it is bound by LDS read throughput for that access pattern "
+                              "(roofline.lds.pattern_frac)")},
     }
     ops = cfg.n_dm * cfg.n_steps * n_pix * (cfg.n_sub + 3 * len(cfg.widths) + 4)
     out["roofline"]["valu_tops"] = ops / (dd_ms * 1e-3) / 1e12
-    # LDS reads of the float32 subband windows, 4 B per subband term of every dedispersed
-    # sample, against 128 B/clk/CU x 256 CUs x 2.4 GHz
+    # LDS reads of the subband windows, 4 lane bytes per subband term of every dedispersed
+    # sample, against the measured LDS read throughput (16-byte consecutive reads, and
+    # ds_read2_b32 at per-lane addresses: the kernel's own pattern)
     lds_bytes = 4 * cfg.n_sub * cfg.n_dm * cfg.n_steps * n_pix
     lds_tbps = lds_bytes / (dd_ms * 1e-3) / 1e12
     out["roofline"]["lds"] = {"bytes_per_launch": lds_bytes, "achieved_TBps": lds_tbps,
-                              "peak_TBps": LDS_PEAK_TBPS, "frac": lds_tbps / LDS_PEAK_TBPS}
+                              "peak_TBps": LDS_PEAK_TBPS, "frac": lds_tbps / LDS_PEAK_TBPS,
+                              "pattern_peak_TBps": LDS_PATTERN_PEAK_TBPS,
+                              "pattern_frac": lds_tbps / LDS_PATTERN_PEAK_TBPS}
     # every stage against the gfx950 peaks: algorithmic HBM bytes per block / HIP-event stage time,
     # and the correlator's int8 MFMA ops / time against 5.0 POP/s dense
     esz = 2 if args.history == "fp16" else 4
