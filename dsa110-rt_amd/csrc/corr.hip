@@ -52,6 +52,37 @@ __device__ __forceinline__ void store_vis(const CorrParams &P, int step, int co,
     *reinterpret_cast<int2 *>(P.vis + 2 * idx) = v;
 }
 
+// Stage 1: the 4-byte word of (channel ch, antenna, packet pkt_l of output step
+// `step`) read `shift` steps later in the stream delayed by P.delay steps: from
+// the raw block, from the carried tail (the previous block's last P.delay
+// steps) before it, zero outside both (the host keeps shifts in [0, delay]).
+__device__ __forceinline__ uint32_t stage1_word(const CorrParams &P, int step, int pkt_l, int ant, int ch,
+                                                int shift) {
+    const int ppf = P.t_int / 2;
+    const int64_t src = (int64_t)(step + shift - P.delay) * ppf + pkt_l;
+    const int64_t n_tail = (int64_t)P.delay * ppf;
+    if (src >= 0 && src < P.n_pkt)
+        return *reinterpret_cast<const uint32_t *>(P.raw + (((src * P.n_ant + ant) * P.n_chan + ch) << 2));
+    if (src < 0 && src >= -n_tail)
+        return *reinterpret_cast<const uint32_t *>(
+            P.tail + ((((src + n_tail) * P.n_ant + ant) * P.n_chan + ch) << 2));
+    return 0u;
+}
+
+// The same for 4 channels ch0 .. ch0 + 3 sharing one shift: one 16-byte load.
+__device__ __forceinline__ uint4 stage1_word4(const CorrParams &P, int step, int pkt_l, int ant, int ch0,
+                                              int shift) {
+    const int ppf = P.t_int / 2;
+    const int64_t src = (int64_t)(step + shift - P.delay) * ppf + pkt_l;
+    const int64_t n_tail = (int64_t)P.delay * ppf;
+    if (src >= 0 && src < P.n_pkt)
+        return *reinterpret_cast<const uint4 *>(P.raw + (((src * P.n_ant + ant) * P.n_chan + ch0) << 2));
+    if (src < 0 && src >= -n_tail)
+        return *reinterpret_cast<const uint4 *>(
+            P.tail + ((((src + n_tail) * P.n_ant + ant) * P.n_chan + ch0) << 2));
+    return make_uint4(0u, 0u, 0u, 0u);
+}
+
 __global__ __launch_bounds__(CORR_THREADS) void corr_i8_kernel(CorrParams P) {
     extern __shared__ __align__(16) int8_t lds[];
     const int tid = threadIdx.x;
@@ -81,8 +112,24 @@ __global__ __launch_bounds__(CORR_THREADS) void corr_i8_kernel(CorrParams P) {
             int grp = grp0, ant = ant0, pkt_l = pkl0;
             while (pkt_l < ppf) {
                 const int pkt = step * ppf + pkt_l;
-                const uint4 w4 = *reinterpret_cast<const uint4 *>(
-                    P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + co * P.csf + 4 * grp) << 2));
+                const int ch0 = co * P.csf + 4 * grp;
+                uint4 w4;
+                if (P.chan_shift == nullptr) {
+                    w4 = *reinterpret_cast<const uint4 *>(P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + ch0) << 2));
+                } else {
+                    // stage 1: one 16-byte load when the 4 channels share their shift
+                    // (channel groups of 4 or more), else one word per channel
+                    const int s0 = P.chan_shift[ch0], s1 = P.chan_shift[ch0 + 1];
+                    const int s2 = P.chan_shift[ch0 + 2], s3 = P.chan_shift[ch0 + 3];
+                    if (s0 == s1 && s0 == s2 && s0 == s3) {
+                        w4 = stage1_word4(P, step, pkt_l, ant, ch0, s0);
+                    } else {
+                        w4.x = stage1_word(P, step, pkt_l, ant, ch0, s0);
+                        w4.y = stage1_word(P, step, pkt_l, ant, ch0 + 1, s1);
+                        w4.z = stage1_word(P, step, pkt_l, ant, ch0 + 2, s2);
+                        w4.w = stage1_word(P, step, pkt_l, ant, ch0 + 3, s3);
+                    }
+                }
                 const uint32_t wv[4] = {w4.x, w4.y, w4.z, w4.w};
                 uint32_t r0[2], r1[2], m0[2], m1[2];
 #pragma unroll
@@ -133,16 +180,7 @@ __global__ __launch_bounds__(CORR_THREADS) void corr_i8_kernel(CorrParams P) {
                         P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + ch) << 2));
                 } else {
                     // stage 1: this channel's samples of step t + shift - delay
-                    const int64_t src = (int64_t)(step + P.chan_shift[ch] - P.delay) * ppf + pkt_l;
-                    const int64_t n_tail = (int64_t)P.delay * ppf;
-                    if (src >= 0 && src < P.n_pkt)
-                        word = *reinterpret_cast<const uint32_t *>(
-                            P.raw + (((src * P.n_ant + ant) * P.n_chan + ch) << 2));
-                    else if (src < 0 && src >= -n_tail)
-                        word = *reinterpret_cast<const uint32_t *>(
-                            P.tail + ((((src + n_tail) * P.n_ant + ant) * P.n_chan + ch) << 2));
-                    else
-                        word = 0;    // outside both (the host keeps shifts in [0, delay])
+                    word = stage1_word(P, step, pkt_l, ant, ch, P.chan_shift[ch]);
                 }
                 // bytes: (t0 p0, t0 p1, t1 p0, t1 p1)
                 const uint32_t r = sext_nib4(word & 0x0F0F0F0Fu);
@@ -307,8 +345,8 @@ static int corr_i8(const void *raw, const void *im, const uint8_t *tail, const i
     P.fuse_pol = fuse_pol ? 1 : 0;
     P.in_layout = input_layout;
     P.out_layout = out_layout;
-    P.vec4 = input_layout == 0 && chan_shift == nullptr && csf % 4 == 0 && n_chan % 4 == 0 &&
-             ((uintptr_t)raw & 15) == 0;
+    P.vec4 = input_layout == 0 && csf % 4 == 0 && n_chan % 4 == 0 && ((uintptr_t)raw & 15) == 0 &&
+             ((uintptr_t)tail & 15) == 0;
     P.chan_shift = chan_shift;
     P.tail = tail;
     P.delay = delay;

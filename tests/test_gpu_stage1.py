@@ -25,8 +25,12 @@ def _stage1_cfg(**kw):
     return dcfg.small_node_config().with_(csf=4, n_sub=16, **kw)
 
 
-def test_stage1_visibilities_exact_across_blocks():
-    cfg = _stage1_cfg(n_samp=512, stage1_dm=375.0)
+@pytest.mark.parametrize("stage1_csf,n_sub", [(1, 16), (4, 8)])
+def test_stage1_visibilities_exact_across_blocks(stage1_csf, n_sub):
+    """stage1_csf 1: per-channel shifts (the correlator reads one word per
+    channel); 4: 4-channel groups share a shift (one 16-byte read per group),
+    two groups per 8-channel subband."""
+    cfg = dcfg.small_node_config().with_(csf=4, n_sub=n_sub, n_samp=512, stage1_dm=375.0, stage1_csf=stage1_csf)
     sc = synth.make_scenario(cfg)
     node = SearchNode(cfg, sc.antpos_e, sc.antpos_n)
     o = OracleStream(cfg, sc.antpos_e, sc.antpos_n)
