@@ -22,7 +22,9 @@ constexpr int IMG_MAX_KB = 20;          // <= 40 rows per chunk (2 rows = 4 K va
 constexpr int IMG_TAP_BATCH = 24;
 constexpr int IMG_JT = 2;               // output tiles per column-transform pass
 constexpr int IMG_KB_GROUP = 4;         // column-transform B operands loaded per group of MFMAs
-static_assert(IMG_MAX_KB % IMG_KB_GROUP == 0, "K-block groups must tile IMG_MAX_KB");        // taps per thread with their loads in flight together
+static_assert(IMG_MAX_KB % IMG_KB_GROUP == 0, "K-block groups must tile IMG_MAX_KB");
+// folded column transform: K-blocks of 4 uv rows (IMG_MAX_KB / 2 of them), rounded up to whole groups
+constexpr int IMG_FOLD_KB = ((IMG_MAX_KB / 2 + IMG_KB_GROUP - 1) / IMG_KB_GROUP) * IMG_KB_GROUP;
 constexpr int TAP_CONJ = 1 << 20;
 constexpr int TAP_NEG = 1 << 21;
 constexpr int TAP_CELL = (1 << 20) - 1;
@@ -355,76 +357,86 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
             }
             }
             const int halfN = N >> 1;
-            // ---- 3. column C2R on MFMA: D[i][j] = sum_k A[i][k] B[k][j],
-            //         k = 2 r + part: A = (cos, -sin)(2 pi r i / N), B = c_r (Re, Im) Y[r][j]
-            const int nkb = (2 * rc + 3) / 4;
-            for (int it = wave; it < n_tiles_1d; it += IMG_THREADS / 64) {
+            // ---- 3. column C2R on MFMA, folded.  With theta = 2 pi r i / N,
+            //         C[i][j] = sum_r c_r cos(theta) Re Y[r][j] and S[i][j] = -sum_r c_r sin(theta) Im Y[r][j]
+            //         give out[i] = C + S and out[N - i] = C - S (cos is even and sin odd in i), so
+            //         only the output rows i in [0, N/2] are transformed: two MFMA chains per tile
+            //         (A = cos or -sin, B = c_r Re Y or c_r Im Y, K = 4 uv rows per MFMA), and
+            //         one LDS read of Y[r][j] feeds both
+            // cube mode carries more live registers: one output tile per pass there
+            constexpr int JT = DENSE == 2 ? 1 : IMG_JT;
+            const int nkb = (rc + 3) / 4;
+            const int n_fold = (halfN + 16) / 16;            // i-tiles covering rows [0, N/2]
+            for (int it = wave; it < n_fold; it += IMG_THREADS / 64) {
                 const int i0 = it * 16;
-                float a_op[IMG_MAX_KB];
+                float a_c[IMG_FOLD_KB], a_s[IMG_FOLD_KB];
                 {
                     const int i = i0 + lr;
 #pragma unroll
-                    for (int kb = 0; kb < IMG_MAX_KB; ++kb) {
-                        const int k = kb * 4 + lg;
-                        const int r = k >> 1;
-                        float a = 0.f;
+                    for (int kb = 0; kb < IMG_FOLD_KB; ++kb) {
+                        const int r = kb * 4 + lg;
+                        float c = 0.f, sn = 0.f;
                         if (kb < nkb && r < rc) {
                             const float2 w = tw[((r0 + r) * i) & (N - 1)];
-                            a = (k & 1) ? -w.y : w.x;
+                            c = w.x;
+                            sn = -w.y;
                         }
-                        a_op[kb] = a;
+                        a_c[kb] = c;
+                        a_s[kb] = sn;
                     }
                 }
-                // IMG_JT output tiles per pass: independent accumulator chains interleave,
+                // JT output tiles per pass: independent accumulator chains interleave,
                 // so one tile's MFMA latency hides behind the others'
-                for (int jt = 0; jt < n_tiles_1d; jt += IMG_JT) {
-                    v4f acc2[IMG_JT];
+                for (int jt = 0; jt < n_tiles_1d; jt += JT) {
+                    v4f acc_c[JT], acc_s[JT];
 #pragma unroll
-                    for (int h = 0; h < IMG_JT; ++h) acc2[h] = v4f{0.f, 0.f, 0.f, 0.f};
+                    for (int h = 0; h < JT; ++h) {
+                        acc_c[h] = v4f{0.f, 0.f, 0.f, 0.f};
+                        acc_s[h] = v4f{0.f, 0.f, 0.f, 0.f};
+                    }
                     // B operands in groups of IMG_KB_GROUP: every LDS load of a group is
                     // issued (clamped row and tile, in bounds) before the group's MFMAs
 #pragma unroll
-                    for (int kb0 = 0; kb0 < IMG_MAX_KB; kb0 += IMG_KB_GROUP) {
+                    for (int kb0 = 0; kb0 < IMG_FOLD_KB; kb0 += IMG_KB_GROUP) {
                         if (kb0 < nkb) {
-                            float bv[IMG_JT][IMG_KB_GROUP];
+                            float bc[JT][IMG_KB_GROUP], bs[JT][IMG_KB_GROUP];
 #pragma unroll
-                            for (int h = 0; h < IMG_JT; ++h) {
+                            for (int h = 0; h < JT; ++h) {
                                 const int jc = (jt + h < n_tiles_1d ? jt + h : jt) * 16;
 #pragma unroll
                                 for (int u = 0; u < IMG_KB_GROUP; ++u) {
-                                    const int k = (kb0 + u) * 4 + lg;
-                                    const int r = k >> 1;
+                                    const int r = (kb0 + u) * 4 + lg;
                                     const bool ok = kb0 + u < nkb && r < rc;
                                     const int rr = ok ? r : 0;
                                     const int rg = r0 + rr;
                                     const float cr = ok ? ((rg == 0 || rg == halfN) ? 1.f : 2.f) : 0.f;
                                     const float2 y = H[rr * RS + jc + lr];
-                                    bv[h][u] = cr * ((k & 1) ? y.y : y.x);
+                                    bc[h][u] = cr * y.x;
+                                    bs[h][u] = cr * y.y;
                                 }
                             }
 #pragma unroll
                             for (int u = 0; u < IMG_KB_GROUP; ++u)
                                 if (kb0 + u < nkb) {
 #pragma unroll
-                                    for (int h = 0; h < IMG_JT; ++h)
-                                        acc2[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[kb0 + u], bv[h][u], acc2[h], 0, 0, 0);
+                                    for (int h = 0; h < JT; ++h) {
+                                        acc_c[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_c[kb0 + u], bc[h][u], acc_c[h], 0, 0, 0);
+                                        acc_s[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_s[kb0 + u], bs[h][u], acc_s[h], 0, 0, 0);
+                                    }
                                 }
                         }
                     }
 #pragma unroll
-                    for (int h = 0; h < IMG_JT; ++h) {
+                    for (int h = 0; h < JT; ++h) {
                     if (jt + h >= n_tiles_1d) break;
-                    const int j0 = (jt + h) * 16;
-                    const v4f acc = acc2[h];
-                    // ---- 4. epilogue
-                    const int j = j0 + lr;
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int i = i0 + 4 * lg + v;
+                    const int j = (jt + h) * 16 + lr;
+                    // ---- 4. epilogue: scale, outer ring and DC pixel zeroed, cube mask,
+                    //         chunk partial sums; row i from C + S, row N - i from C - S
+                    auto emit = [&](int i, float x) {
                         const bool inside = DENSE || (i >= P.npad && i < N - P.npad && j >= P.npad &&
                                                         j < N - P.npad && !(i == halfN && j == halfN));
                         const int64_t o = (int64_t)i * N + j;
-                        float val = inside ? acc[v] * out_scale : 0.f;
+                        float val = inside ? x * out_scale : 0.f;
                         if constexpr (DENSE == 2) {
                             if (P.mask != nullptr) val *= P.mask[(int64_t)s * P.mask_stride + o];
                         }
@@ -437,6 +449,12 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                             if (ck > 0) val += ld_as_float(out, o);
                             st_from_float(out, o, val);
                         }
+                    };
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int i = i0 + 4 * lg + v;
+                        if (i <= halfN) emit(i, acc_c[h][v] + acc_s[h][v]);
+                        if (i >= 1 && i < halfN) emit(N - i, acc_c[h][v] - acc_s[h][v]);
                     }
                     }
                 }
