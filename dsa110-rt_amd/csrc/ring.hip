@@ -277,16 +277,21 @@ __global__ __launch_bounds__(256) void static_sky_kernel(T *img, int n_sub, int 
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t s = i / n_pix, p = i - s * n_pix;
         T *col = img + s * (int64_t)n_steps * n_pix + p;
+        // one pass: the block sum of the images as they arrive, and the earlier
+        // blocks' mean subtracted from each
         double acc = 0.0;
-        for (int t = 0; t < n_steps; ++t) acc += (double)ld_as_float(col, (int64_t)t * n_pix);
-        const float mean_now = (float)(acc / (double)n_steps);
         if (subtract && count > 0) {
             const float m = (float)(wsum[i] / (double)count);
             for (int t = 0; t < n_steps; ++t) {
                 const int64_t o = (int64_t)t * n_pix;
-                st_from_float(col, o, ld_as_float(col, o) - m);
+                const float x = ld_as_float(col, o);
+                acc += (double)x;
+                st_from_float(col, o, x - m);
             }
+        } else {
+            for (int t = 0; t < n_steps; ++t) acc += (double)ld_as_float(col, (int64_t)t * n_pix);
         }
+        const float mean_now = (float)(acc / (double)n_steps);
         float *slot = ring + (int64_t)head * n + i;
         double w = wsum[i];
         if (count == window) w -= (double)*slot;
