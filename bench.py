@@ -470,8 +470,12 @@ def main():
     maxt = 8 if plan.n_terms <= 8 else 16 if plan.n_terms <= 16 else 0
     is_async = plan.dma
     tname = "_Float16" if args.history == "fp16" else "float"
-    kname = (f"dedisp_kernel<{tname}, {len(cfg.widths)}, {str(plan.paired).lower()}, {maxt}"
-             + (", true>" if is_async else ">"))
+    # template arguments <T, NW, PAIRED, MAXT, ASYNC, QB, EXACT> of the kept instance: plans with
+    # exactly 16 terms (8 with pair rows) run the pipelined exact-term sums
+    exact = (plan.n_terms == 8) if plan.paired else (plan.n_terms == 16)
+    exact = exact and not plan.rows_bf16
+    kname = (f"dedisp_kernel<{tname}, {len(cfg.widths)}, {str(plan.paired).lower()}, {maxt}, "
+             f"{str(is_async).lower()}, {str(plan.rows_bf16).lower()}, {str(exact).lower()}>")
     out = {
         "metric": "real-time factor + p99 block->candidate latency, 96-ant 384-ch node; chosen vs plain",
         "value": rtf,
