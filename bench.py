@@ -31,10 +31,10 @@ import torch
 HBM_PEAK_GBPS = 8000.0
 # LDS read throughput of the whole GPU, measured by dsa110-rt_amd/csrc/probe/lds_read_probe.hip
 # (lane bytes per second, 256 CUs at the 2.1-2.3 GHz clock held under load): 16-byte reads of
-# consecutive words 135.6 TB/s (243 B/clk/CU); ds_read2_b32 at clustered per-lane addresses --
-# the dedispersion kernel's window reads, neighbouring DMs a few words apart -- 67.5 TB/s
-LDS_PEAK_TBPS = 135.6
-LDS_PATTERN_PEAK_TBPS = 67.5
+# consecutive words 135.3 TB/s (243 B/clk/CU); ds_read2_b32 at clustered per-lane addresses --
+# the dedispersion kernel's window reads, neighbouring DMs a few words apart -- 67.9 TB/s
+LDS_PEAK_TBPS = 135.3
+LDS_PATTERN_PEAK_TBPS = 67.9
 INT8_PEAK_POPS = 5.0                           # MI355X dense int8 MFMA
 MFMA_I8_PEAK_TOPS = 5000.0
 FP32_PEAK_TFLOPS = 157.3

@@ -19,7 +19,7 @@ constexpr int MASK = NF - 1 - 1024;       // base indices stay below NF - 1024
 constexpr int ITERS = 4096;
 
 enum Pattern { CONSEC_B32, BROADCAST_B32, PERLANE_B32, CLUSTER_B32, CONSEC_B128, PERLANE_B64, PERLANE_B128,
-               CLUSTER_B128 };
+               CLUSTER_B128, CLUSTER_B64 };
 
 template <int P>
 __global__ __launch_bounds__(256) void lds_read(float *out, long long *clk, int salt) {
@@ -45,8 +45,8 @@ __global__ __launch_bounds__(256) void lds_read(float *out, long long *clk, int 
             const int b = (it * 256 + lane * 4) & MASK & ~3;
             const float4 v = *reinterpret_cast<const float4 *>(&s[b]);
             a0 += v.x; a1 += v.y; a2 += v.z; a3 += v.w;
-        } else if constexpr (P == PERLANE_B64) {
-            const int b = ((spread & ~1) + it * 4) & MASK & ~1;
+        } else if constexpr (P == PERLANE_B64 || P == CLUSTER_B64) {
+            const int b = (((P == PERLANE_B64 ? spread : cluster) & ~1) + it * 4) & MASK & ~1;
             const float2 u = *reinterpret_cast<const float2 *>(&s[b]);
             const float2 v = *reinterpret_cast<const float2 *>(&s[b + 2]);
             a0 += u.x; a1 += u.y; a2 += v.x; a3 += v.y;
@@ -122,6 +122,7 @@ int main() {
     run<PERLANE_B64>("b64, per-lane bases", n_cu, nblk, out, clk);
     run<PERLANE_B128>("b128, per-lane bases", n_cu, nblk, out, clk);
     run<CLUSTER_B128>("b128, clustered bases", n_cu, nblk, out, clk);
+    run<CLUSTER_B64>("b64, clustered bases", n_cu, nblk, out, clk);
     CHECK(hipFree(out));
     CHECK(hipFree(clk));
     return 0;
