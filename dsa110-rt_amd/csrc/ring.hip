@@ -175,6 +175,45 @@ __global__ __launch_bounds__(256) void downsample_time_kernel(const T *in, T *ou
     }
 }
 
+// Static-sky subtraction on a block of images img[s][t][p] (the image-domain
+// form of StaticSkyMean, src/dsart/services/corr_fast_integration.py:1049-1246:
+// imaging is linear, so subtracting a per-cell mean grid from every step
+// subtracts its image).  Per (subband, pixel): the block mean of the images
+// (float64 sum); when `subtract`, every step loses the mean of the previous
+// blocks' means, wsum / count; then this block's mean enters the window
+// (ring[head], evicting the oldest when the window is full).
+template <typename T>
+__global__ __launch_bounds__(256) void static_sky_kernel(T *img, int n_sub, int n_steps, int64_t n_pix,
+                                                         float *ring, double *wsum, int window, int head,
+                                                         int count, int subtract) {
+    const int64_t n = (int64_t)n_sub * n_pix;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = i / n_pix, p = i - s * n_pix;
+        T *col = img + s * (int64_t)n_steps * n_pix + p;
+        // one pass: the block sum of the images as they arrive, and the earlier
+        // blocks' mean subtracted from each
+        double acc = 0.0;
+        if (subtract && count > 0) {
+            const float m = (float)(wsum[i] / (double)count);
+            for (int t = 0; t < n_steps; ++t) {
+                const int64_t o = (int64_t)t * n_pix;
+                const float x = ld_as_float(col, o);
+                acc += (double)x;
+                st_from_float(col, o, x - m);
+            }
+        } else {
+            for (int t = 0; t < n_steps; ++t) acc += (double)ld_as_float(col, (int64_t)t * n_pix);
+        }
+        const float mean_now = (float)(acc / (double)n_steps);
+        float *slot = ring + (int64_t)head * n + i;
+        double w = wsum[i];
+        if (count == window) w -= (double)*slot;
+        *slot = mean_now;
+        wsum[i] = w + (double)mean_now;
+    }
+}
+
 }  // namespace dsx
 
 using namespace dsx;
@@ -259,45 +298,6 @@ extern "C" int dsx_downsample_time(const void *in, void *out, int n_planes, int 
         downsample_time_kernel<float><<<blocks, 256, 0, s>>>((const float *)in, (float *)out, total, n_in,
                                                               n_inner, factor);
     return check_launch("dsx_downsample_time");
-}
-
-// Static-sky subtraction on a block of images img[s][t][p] (the image-domain
-// form of StaticSkyMean, src/dsart/services/corr_fast_integration.py:1049-1246:
-// imaging is linear, so subtracting a per-cell mean grid from every step
-// subtracts its image).  Per (subband, pixel): the block mean of the images
-// (float64 sum); when `subtract`, every step loses the mean of the previous
-// blocks' means, wsum / count; then this block's mean enters the window
-// (ring[head], evicting the oldest when the window is full).
-template <typename T>
-__global__ __launch_bounds__(256) void static_sky_kernel(T *img, int n_sub, int n_steps, int64_t n_pix,
-                                                         float *ring, double *wsum, int window, int head,
-                                                         int count, int subtract) {
-    const int64_t n = (int64_t)n_sub * n_pix;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = i / n_pix, p = i - s * n_pix;
-        T *col = img + s * (int64_t)n_steps * n_pix + p;
-        // one pass: the block sum of the images as they arrive, and the earlier
-        // blocks' mean subtracted from each
-        double acc = 0.0;
-        if (subtract && count > 0) {
-            const float m = (float)(wsum[i] / (double)count);
-            for (int t = 0; t < n_steps; ++t) {
-                const int64_t o = (int64_t)t * n_pix;
-                const float x = ld_as_float(col, o);
-                acc += (double)x;
-                st_from_float(col, o, x - m);
-            }
-        } else {
-            for (int t = 0; t < n_steps; ++t) acc += (double)ld_as_float(col, (int64_t)t * n_pix);
-        }
-        const float mean_now = (float)(acc / (double)n_steps);
-        float *slot = ring + (int64_t)head * n + i;
-        double w = wsum[i];
-        if (count == window) w -= (double)*slot;
-        *slot = mean_now;
-        wsum[i] = w + (double)mean_now;
-    }
 }
 
 extern "C" int dsx_static_sky(void *img, int img_dtype, int n_sub, int n_steps, int64_t n_pix, float *ring,
