@@ -20,7 +20,7 @@ namespace dsx {
 constexpr int IMG_THREADS = 256;
 constexpr int IMG_MAX_KB = 20;          // <= 40 rows per chunk (2 rows = 4 K values per MFMA)
 constexpr int IMG_TAP_BATCH = 24;
-constexpr int IMG_JT = 2;               // output tiles per column-transform pass
+constexpr int IMG_JT = 4;               // output tiles per column-transform pass
 constexpr int IMG_KB_GROUP = 4;         // column-transform B operands loaded per group of MFMAs
 static_assert(IMG_MAX_KB % IMG_KB_GROUP == 0, "K-block groups must tile IMG_MAX_KB");
 // folded column transform: K-blocks of 4 uv rows (IMG_MAX_KB / 2 of them), rounded up to whole groups
@@ -363,11 +363,15 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
             //         only the output rows i in [0, N/2] are transformed: two MFMA chains per tile
             //         (A = cos or -sin, B = c_r Re Y or c_r Im Y, K = 4 uv rows per MFMA), and
             //         one LDS read of Y[r][j] feeds both
-            // cube mode carries more live registers: one output tile per pass there
-            constexpr int JT = DENSE == 2 ? 1 : IMG_JT;
+            constexpr int JT = IMG_JT;
             const int nkb = (rc + 3) / 4;
             const int n_fold = (halfN + 16) / 16;            // i-tiles covering rows [0, N/2]
-            for (int it = wave; it < n_fold; it += IMG_THREADS / 64) {
+            // work units (i-tile, group of JT j-tiles) spread over the waves, so the
+            // N/32 + 1 folded i-tiles leave no wave idle
+            const int n_jg = (n_tiles_1d + JT - 1) / JT;
+            for (int unit = wave; unit < n_fold * n_jg; unit += IMG_THREADS / 64) {
+                const int it = unit / n_jg;
+                const int jt = (unit - it * n_jg) * JT;
                 const int i0 = it * 16;
                 float a_c[IMG_FOLD_KB], a_s[IMG_FOLD_KB];
                 {
@@ -385,9 +389,9 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                         a_s[kb] = sn;
                     }
                 }
-                // JT output tiles per pass: independent accumulator chains interleave,
+                // JT output tiles per unit: independent accumulator chains interleave,
                 // so one tile's MFMA latency hides behind the others'
-                for (int jt = 0; jt < n_tiles_1d; jt += JT) {
+                {
                     v4f acc_c[JT], acc_s[JT];
 #pragma unroll
                     for (int h = 0; h < JT; ++h) {
