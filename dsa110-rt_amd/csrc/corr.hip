@@ -38,6 +38,8 @@ struct CorrParams {
     int delay;
 };
 
+constexpr int CORR_PF = 12;   // prefetched 16-byte raw words per thread and item
+
 __device__ __forceinline__ void store_vis(const CorrParams &P, int step, int co, int q, int bls,
                                           int re, int im) {
     const int npol_out = P.fuse_pol ? 1 : 2;
@@ -103,59 +105,87 @@ __global__ __launch_bounds__(CORR_THREADS) void corr_i8_kernel(CorrParams P) {
     const int grp0 = P.vec4 ? tid % q4 : 0, row0 = P.vec4 ? tid / q4 : 0;
     const int ant0 = P.vec4 ? row0 % P.n_ant : 0, pkl0 = P.vec4 ? row0 / P.n_ant : 0;
 
+    // the 16 raw bytes (4 channels x 2 steps x 2 pols) of one slot of output step `step`
+    auto load_slot = [&](int step, int co, int grp, int ant, int pkt_l) -> uint4 {
+        const int ch0 = co * P.csf + 4 * grp;
+        if (P.chan_shift == nullptr) {
+            const int pkt = step * ppf + pkt_l;
+            return *reinterpret_cast<const uint4 *>(P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + ch0) << 2));
+        }
+        // stage 1: one 16-byte load when the 4 channels share their shift
+        // (channel groups of 4 or more), else one word per channel
+        const int s0 = P.chan_shift[ch0], s1 = P.chan_shift[ch0 + 1];
+        const int s2 = P.chan_shift[ch0 + 2], s3 = P.chan_shift[ch0 + 3];
+        if (s0 == s1 && s0 == s2 && s0 == s3) return stage1_word4(P, step, pkt_l, ant, ch0, s0);
+        return make_uint4(stage1_word(P, step, pkt_l, ant, ch0, s0), stage1_word(P, step, pkt_l, ant, ch0 + 1, s1),
+                          stage1_word(P, step, pkt_l, ant, ch0 + 2, s2), stage1_word(P, step, pkt_l, ant, ch0 + 3, s3));
+    };
+    // sign-extend the nibbles of one slot and store them into the antenna's LDS
+    // rows; per part 8 bytes (4 channels x 2 steps) per LDS store
+    auto unpack_slot = [&](const uint4 w4, int grp, int ant, int pkt_l) {
+        const uint32_t wv[4] = {w4.x, w4.y, w4.z, w4.w};
+        uint32_t r0[2], r1[2], m0[2], m1[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            uint32_t pr0[2], pr1[2], pm0[2], pm1[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const uint32_t word = wv[2 * h + c];      // bytes: (t0 p0, t0 p1, t1 p0, t1 p1)
+                const uint32_t r = sext_nib4(word & 0x0F0F0F0Fu);
+                const uint32_t m = sext_nib4((word >> 4) & 0x0F0F0F0Fu);
+                pr0[c] = (r & 0xFFu) | ((r >> 8) & 0xFF00u);
+                pr1[c] = ((r >> 8) & 0xFFu) | ((r >> 16) & 0xFF00u);
+                pm0[c] = (m & 0xFFu) | ((m >> 8) & 0xFF00u);
+                pm1[c] = ((m >> 8) & 0xFFu) | ((m >> 16) & 0xFF00u);
+            }
+            r0[h] = pr0[0] | (pr0[1] << 16);
+            r1[h] = pr1[0] | (pr1[1] << 16);
+            m0[h] = pm0[0] | (pm0[1] << 16);
+            m1[h] = pm1[0] | (pm1[1] << 16);
+        }
+        const int k = (pkt_l * P.csf + 4 * grp) * 2;           // 8-byte aligned
+        int8_t *row = lds + ant * P.row_stride;
+        *reinterpret_cast<uint2 *>(row + k) = make_uint2(r0[0], r0[1]);
+        *reinterpret_cast<uint2 *>(row + kq + k) = make_uint2(r1[0], r1[1]);
+        *reinterpret_cast<uint2 *>(row + 2 * kq + k) = make_uint2(m0[0], m0[1]);
+        *reinterpret_cast<uint2 *>(row + 3 * kq + k) = make_uint2(m1[0], m1[1]);
+    };
+    // prefetch: when an item's slots fit CORR_PF per thread, the next item's raw
+    // words are loaded into registers while the current item's MFMAs run
+    const int n_slot = P.vec4 ? ppf * P.n_ant * q4 : 0;
+    const bool pf = P.vec4 && n_slot <= CORR_PF * CORR_THREADS;
+    uint4 wbuf[CORR_PF];
+    auto fetch = [&](int it_f) {
+        const int st_f = it_f / P.n_cout;
+        const int co_f = it_f - st_f * P.n_cout;
+#pragma unroll
+        for (int k = 0; k < CORR_PF; ++k) {
+            const int sl = tid + k * CORR_THREADS;
+            if (sl < n_slot) {
+                const int row = sl / q4;
+                wbuf[k] = load_slot(st_f, co_f, sl - row * q4, row % P.n_ant, row / P.n_ant);
+            }
+        }
+    };
+    if (pf && (int)blockIdx.x < P.n_items) fetch(blockIdx.x);
+
     for (int item = blockIdx.x; item < P.n_items; item += gridDim.x) {
         const int step = item / P.n_cout;
         const int co = item - step * P.n_cout;
         // ---- unpack this (step, channel group) into LDS rows
-        if (P.vec4) {
-            // 4 channels per 16-byte load; per part 8 bytes (4 channels x 2 steps) per LDS store
+        if (pf) {
+#pragma unroll
+            for (int k = 0; k < CORR_PF; ++k) {
+                const int sl = tid + k * CORR_THREADS;
+                if (sl < n_slot) {
+                    const int row = sl / q4;
+                    unpack_slot(wbuf[k], sl - row * q4, row % P.n_ant, row / P.n_ant);
+                }
+            }
+        } else if (P.vec4) {
             int grp = grp0, ant = ant0, pkt_l = pkl0;
             while (pkt_l < ppf) {
-                const int pkt = step * ppf + pkt_l;
-                const int ch0 = co * P.csf + 4 * grp;
-                uint4 w4;
-                if (P.chan_shift == nullptr) {
-                    w4 = *reinterpret_cast<const uint4 *>(P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + ch0) << 2));
-                } else {
-                    // stage 1: one 16-byte load when the 4 channels share their shift
-                    // (channel groups of 4 or more), else one word per channel
-                    const int s0 = P.chan_shift[ch0], s1 = P.chan_shift[ch0 + 1];
-                    const int s2 = P.chan_shift[ch0 + 2], s3 = P.chan_shift[ch0 + 3];
-                    if (s0 == s1 && s0 == s2 && s0 == s3) {
-                        w4 = stage1_word4(P, step, pkt_l, ant, ch0, s0);
-                    } else {
-                        w4.x = stage1_word(P, step, pkt_l, ant, ch0, s0);
-                        w4.y = stage1_word(P, step, pkt_l, ant, ch0 + 1, s1);
-                        w4.z = stage1_word(P, step, pkt_l, ant, ch0 + 2, s2);
-                        w4.w = stage1_word(P, step, pkt_l, ant, ch0 + 3, s3);
-                    }
-                }
-                const uint32_t wv[4] = {w4.x, w4.y, w4.z, w4.w};
-                uint32_t r0[2], r1[2], m0[2], m1[2];
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    uint32_t pr0[2], pr1[2], pm0[2], pm1[2];
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        const uint32_t word = wv[2 * h + c];      // bytes: (t0 p0, t0 p1, t1 p0, t1 p1)
-                        const uint32_t r = sext_nib4(word & 0x0F0F0F0Fu);
-                        const uint32_t m = sext_nib4((word >> 4) & 0x0F0F0F0Fu);
-                        pr0[c] = (r & 0xFFu) | ((r >> 8) & 0xFF00u);
-                        pr1[c] = ((r >> 8) & 0xFFu) | ((r >> 16) & 0xFF00u);
-                        pm0[c] = (m & 0xFFu) | ((m >> 8) & 0xFF00u);
-                        pm1[c] = ((m >> 8) & 0xFFu) | ((m >> 16) & 0xFF00u);
-                    }
-                    r0[h] = pr0[0] | (pr0[1] << 16);
-                    r1[h] = pr1[0] | (pr1[1] << 16);
-                    m0[h] = pm0[0] | (pm0[1] << 16);
-                    m1[h] = pm1[0] | (pm1[1] << 16);
-                }
-                const int k = (pkt_l * P.csf + 4 * grp) * 2;           // 8-byte aligned
-                int8_t *row = lds + ant * P.row_stride;
-                *reinterpret_cast<uint2 *>(row + k) = make_uint2(r0[0], r0[1]);
-                *reinterpret_cast<uint2 *>(row + kq + k) = make_uint2(r1[0], r1[1]);
-                *reinterpret_cast<uint2 *>(row + 2 * kq + k) = make_uint2(m0[0], m0[1]);
-                *reinterpret_cast<uint2 *>(row + 3 * kq + k) = make_uint2(m1[0], m1[1]);
+                unpack_slot(load_slot(step, co, grp, ant, pkt_l), grp, ant, pkt_l);
                 grp += d_grp;
                 const int carry = grp >= q4 ? 1 : 0;
                 grp -= carry * q4;
@@ -217,6 +247,7 @@ __global__ __launch_bounds__(CORR_THREADS) void corr_i8_kernel(CorrParams P) {
             }
         }
         __syncthreads();
+        if (pf && item + (int)gridDim.x < P.n_items) fetch(item + gridDim.x);
 
         // ---- MFMA over upper-triangle tile pairs (b tile <= a tile)
         for (int pair = wave; pair < P.n_pairs; pair += CORR_THREADS / 64) {
