@@ -26,6 +26,7 @@
 namespace dsx {
 
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
 constexpr int CCAL_THREADS = 256;
 constexpr int CCAL_WAVES = CCAL_THREADS / 64;
@@ -42,6 +43,7 @@ struct CalCorrParams {
     float scale;
     int n_pkt, n_ant, n_chan, t_int, csf, fuse_pol, in_layout, out_layout, n_acc;
     int n_steps, n_cout, nbase, csf_c, n_chunks, kc, row_stride, n_tiles, n_pairs, n_items;
+    int vec4;                    // raw input read 4 channels (16 bytes) at a time: csf_c, n_chan multiples of 4
 };
 
 // x = v * g in float32, products and sums rounded separately
@@ -103,7 +105,44 @@ __global__ __launch_bounds__(CCAL_THREADS) void corr_cal_kernel(CalCorrParams P)
 
         for (int chunk = 0; chunk < P.n_chunks; ++chunk) {
             const int ch0 = co * P.csf + chunk * P.csf_c;
-            if (P.in_layout == 0) {
+            if (P.vec4) {
+                // one 16-byte load per (packet, antenna, 4-channel group); each channel's
+                // two steps stored as one fp16 pair per part
+                const int g4 = P.csf_c >> 2;
+                const int n_slot = ppf * P.n_ant * g4;
+                for (int sl = tid; sl < n_slot; sl += CCAL_THREADS) {
+                    const int grp = sl % g4;
+                    const int rest = sl / g4;
+                    const int ant = rest % P.n_ant;
+                    const int pkt_l = rest / P.n_ant;
+                    const int pkt = step * ppf + pkt_l;
+                    const int chb = ch0 + 4 * grp;
+                    const uint4 w4 = *reinterpret_cast<const uint4 *>(
+                        P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + chb) << 2));
+                    const uint32_t wv[4] = {w4.x, w4.y, w4.z, w4.w};
+                    _Float16 *row = lds16 + ant * RS;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const uint32_t word = wv[c];                       // (t0 p0, t0 p1, t1 p0, t1 p1)
+                        const int ch = chb + c;
+                        const int k = (pkt_l * P.csf_c + 4 * grp + c) * 2;
+#pragma unroll
+                        for (int p = 0; p < 2; ++p) {
+                            const uint32_t b0 = (word >> (8 * p)) & 0xFFu;         // t0
+                            const uint32_t b1 = (word >> (8 * (2 + p))) & 0xFFu;   // t1
+                            const float vr0 = (float)(_Float16)(nib_lo(b0) * P.scale);
+                            const float vi0 = (float)(_Float16)(nib_hi(b0) * P.scale);
+                            const float vr1 = (float)(_Float16)(nib_lo(b1) * P.scale);
+                            const float vi1 = (float)(_Float16)(nib_hi(b1) * P.scale);
+                            _Float16 xr0, xi0, xr1, xi1;
+                            cal_operand(P, ch, p, ant, pkt, vr0, vi0, xr0, xi0);
+                            cal_operand(P, ch, p, ant, pkt, vr1, vi1, xr1, xi1);
+                            *reinterpret_cast<h2 *>(row + p * kc + k) = h2{xr0, xr1};
+                            *reinterpret_cast<h2 *>(row + 2 * kc + p * kc + k) = h2{xi0, xi1};
+                        }
+                    }
+                }
+            } else if (P.in_layout == 0) {
                 // one 4-byte word (t0 p0, t0 p1, t1 p0, t1 p1) per (packet, antenna, channel)
                 const int n_words = ppf * P.n_ant * P.csf_c;
                 for (int w = tid; w < n_words; w += CCAL_THREADS) {
@@ -128,10 +167,8 @@ __global__ __launch_bounds__(CCAL_THREADS) void corr_cal_kernel(CalCorrParams P)
                         _Float16 xr0, xi0, xr1, xi1;
                         cal_operand(P, ch, p, ant, pkt, vr0, vi0, xr0, xi0);
                         cal_operand(P, ch, p, ant, pkt, vr1, vi1, xr1, xi1);
-                        row[p * kc + k] = xr0;
-                        row[p * kc + k + 1] = xr1;
-                        row[2 * kc + p * kc + k] = xi0;
-                        row[2 * kc + p * kc + k + 1] = xi1;
+                        *reinterpret_cast<h2 *>(row + p * kc + k) = h2{xr0, xr1};
+                        *reinterpret_cast<h2 *>(row + 2 * kc + p * kc + k) = h2{xi0, xi1};
                     }
                 }
             } else {
@@ -321,6 +358,7 @@ extern "C" int dsx_corr_cal(const void *volt_re, const void *volt_im, const floa
     if (lds > 160 * 1024)
         return set_error(DSX_ERANGE, "dsx_corr_cal: %zu B of LDS needed (t_int too large)", lds);
     P.n_chunks = csf / P.csf_c;
+    P.vec4 = input_layout == 0 && P.csf_c % 4 == 0 && n_chan % 4 == 0 && ((uintptr_t)volt_re & 15) == 0;
     hipStream_t s = as_stream(stream);
     return P.fuse_pol ? launch_cal_q<1>(P, lds, s) : launch_cal_q<2>(P, lds, s);
 }
