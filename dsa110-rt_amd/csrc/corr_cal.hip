@@ -110,6 +110,49 @@ __global__ __launch_bounds__(CCAL_THREADS) void corr_cal_kernel(CalCorrParams P)
                 // two steps stored as one fp16 pair per part
                 const int g4 = P.csf_c >> 2;
                 const int n_slot = ppf * P.n_ant * g4;
+                if (P.cal_re != nullptr && P.tkeep == nullptr) {
+                    // gains without a time mask: a slot's 8 complex gains and its raw
+                    // bytes are loaded together, then the arithmetic of cal_operand
+                    for (int sl = tid; sl < n_slot; sl += CCAL_THREADS) {
+                        const int grp = sl % g4;
+                        const int rest = sl / g4;
+                        const int ant = rest % P.n_ant;
+                        const int pkt_l = rest / P.n_ant;
+                        const int pkt = step * ppf + pkt_l;
+                        const int chb = ch0 + 4 * grp;
+                        float gr[4][2], gim[4][2];
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+#pragma unroll
+                            for (int p = 0; p < 2; ++p) {
+                                const int64_t gi_idx = ((int64_t)(chb + c) * 2 + p) * P.n_ant + ant;
+                                gr[c][p] = P.cal_re[gi_idx];
+                                gim[c][p] = P.cal_im[gi_idx];
+                            }
+                        const uint4 w4 = *reinterpret_cast<const uint4 *>(
+                            P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + chb) << 2));
+                        const uint32_t wv[4] = {w4.x, w4.y, w4.z, w4.w};
+                        _Float16 *row = lds16 + ant * RS;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            const uint32_t word = wv[c];
+                            const int k = (pkt_l * P.csf_c + 4 * grp + c) * 2;
+#pragma unroll
+                            for (int p = 0; p < 2; ++p) {
+                                const uint32_t b0 = (word >> (8 * p)) & 0xFFu;
+                                const uint32_t b1 = (word >> (8 * (2 + p))) & 0xFFu;
+                                const float vr0 = (float)(_Float16)(nib_lo(b0) * P.scale);
+                                const float vi0 = (float)(_Float16)(nib_hi(b0) * P.scale);
+                                const float vr1 = (float)(_Float16)(nib_lo(b1) * P.scale);
+                                const float vi1 = (float)(_Float16)(nib_hi(b1) * P.scale);
+                                const float2 x0 = cal_mul(vr0, vi0, gr[c][p], gim[c][p]);
+                                const float2 x1 = cal_mul(vr1, vi1, gr[c][p], gim[c][p]);
+                                *reinterpret_cast<h2 *>(row + p * kc + k) = h2{(_Float16)x0.x, (_Float16)x1.x};
+                                *reinterpret_cast<h2 *>(row + 2 * kc + p * kc + k) = h2{(_Float16)x0.y, (_Float16)x1.y};
+                            }
+                        }
+                    }
+                } else
                 for (int sl = tid; sl < n_slot; sl += CCAL_THREADS) {
                     const int grp = sl % g4;
                     const int rest = sl / g4;
