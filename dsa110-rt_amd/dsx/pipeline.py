@@ -782,15 +782,25 @@ class MultiRangeNode:
         img = self.front_node.front(raw, n_samp, 0, s)
         N2 = cfg.n_grid * cfg.n_grid
         pends = []
+        done = {1: img}                     # images at each resolution made so far
         for nd, f, off, buf in self.ranges:
             if n_steps % (DETECT_CHUNK_STEPS * f):
                 raise ValueError(f"{n_steps} steps do not downsample by {f} into whole 64-sample chunks")
             if f == 1:
                 img_f = img
             else:
-                img_f = buf[:cfg.n_sub * (n_steps // f) * N2].view(cfg.n_sub, n_steps // f, cfg.n_grid, cfg.n_grid)
-                _lib.call("dsx_downsample_time", _lib.ptr(img), _lib.ptr(img_f), cfg.n_sub, n_steps, N2, f,
-                          _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32, _lib.stream(s))
+                # from the coarsest resolution already made whose factor divides f
+                # (a 4x range sums pairs of the 2x range's images)
+                fs = max(k for k in done if f % k == 0)
+                if fs == f:
+                    img_f = done[f]          # an earlier range at the same resolution
+                else:
+                    img_f = buf[:cfg.n_sub * (n_steps // f) * N2].view(cfg.n_sub, n_steps // f, cfg.n_grid,
+                                                                       cfg.n_grid)
+                    _lib.call("dsx_downsample_time", _lib.ptr(done[fs]), _lib.ptr(img_f), cfg.n_sub, n_steps // fs,
+                              N2, f // fs, _lib.DSX_F16 if self.ring_dtype == torch.float16 else _lib.DSX_F32,
+                              _lib.stream(s))
+                    done[f] = img_f
             nd.push_images(img_f, n_steps // f, s)
             pends.append(nd.detect_async(n_steps // f, 0, s))
         recs, T0 = [], None

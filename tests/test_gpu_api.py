@@ -192,6 +192,29 @@ def test_ring_advance_matches_numpy(dtype, hist):
         assert np.array_equal(got[:, :, :hist + n_new], want[:, :, :hist + n_new]), f"block of {n_new} steps"
 
 
+@pytest.mark.parametrize("dtype,n_inner,f", [(torch.float32, 256, 2), (torch.float32, 256, 4), (torch.float32, 6, 2),
+                                             (torch.float16, 256, 4)])
+def test_downsample_time_matches_numpy(dtype, n_inner, f):
+    """dsx_downsample_time: out[s][t][k] = sum_j in[s][f t + j][k] accumulated in
+    float32 in j order, stored in the input dtype -- bit-exact against the
+    same arithmetic in numpy.  n_inner 256 (float32) takes the 16-byte path,
+    6 the element path."""
+    from dsx import _lib
+    n_planes, n_in = 3, 64
+    rng = np.random.default_rng(9)
+    x = torch.from_numpy(rng.standard_normal((n_planes, n_in, n_inner)).astype(np.float32)).to(dtype).cuda()
+    out = torch.empty((n_planes, n_in // f, n_inner), dtype=dtype, device="cuda")
+    code = _lib.DSX_F16 if dtype == torch.float16 else _lib.DSX_F32
+    _lib.call("dsx_downsample_time", _lib.ptr(x), _lib.ptr(out), n_planes, n_in, n_inner, f, code, None)
+    torch.cuda.synchronize()
+    xs = x.float().cpu().numpy().reshape(n_planes, n_in // f, f, n_inner)
+    acc = np.zeros((n_planes, n_in // f, n_inner), np.float32)
+    for j in range(f):
+        acc = (acc + xs[:, :, j]).astype(np.float32)
+    want = acc.astype(np.float16).astype(np.float32) if dtype == torch.float16 else acc
+    assert np.array_equal(out.float().cpu().numpy(), want)
+
+
 @pytest.mark.parametrize("dtype,row_len", [(torch.float32, 2300), (torch.float16, 4500)])
 def test_dedisp_prime_first_block_means(dtype, row_len):
     """dsx_dedisp_prime: for DMs without state the written mean is the float64
