@@ -31,6 +31,7 @@ typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 constexpr int CCAL_THREADS = 256;
 constexpr int CCAL_WAVES = CCAL_THREADS / 64;
 constexpr int CCAL_LDS_BUDGET = 80 * 1024;   // two work-groups per CU
+constexpr int CCAL_PF = 6;                   // prefetched 16-byte raw words per thread and chunk
 
 struct CalCorrParams {
     const uint8_t *raw;          // in_layout 0: on-wire (n_pkt, n_ant, n_chan, 2t, 2p) bytes
@@ -92,6 +93,100 @@ __global__ __launch_bounds__(CCAL_THREADS) void corr_cal_kernel(CalCorrParams P)
     for (int i = tid; i < n_vec; i += CCAL_THREADS) reinterpret_cast<v4i *>(lds16)[i] = v4i{0, 0, 0, 0};
     __syncthreads();
 
+    // vec4 input: slot sl = (packet, antenna, 4-channel group) of a chunk, one
+    // 16-byte load of raw bytes each
+    const int g4 = P.csf_c >> 2;
+    const int n_slot = P.vec4 ? ppf * P.n_ant * g4 : 0;
+    const bool gain_fast = P.cal_re != nullptr && P.tkeep == nullptr;
+    auto load_slot = [&](int sl, int ch0, int step) -> uint4 {
+        asm volatile("" : "+v"(sl));   // keep the slot's index math out of the item loop
+        const int grp = sl % g4;
+        const int rest = sl / g4;
+        const int ant = rest % P.n_ant;
+        const int pkt = step * ppf + rest / P.n_ant;
+        return *reinterpret_cast<const uint4 *>(
+            P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + ch0 + 4 * grp) << 2));
+    };
+    // one slot's operands into the antenna's LDS row; each channel's two steps
+    // stored as one fp16 pair per part
+    auto store_slot = [&](const uint4 w4, int sl, int ch0, int step) {
+        asm volatile("" : "+v"(sl));
+        const int grp = sl % g4;
+        const int rest = sl / g4;
+        const int ant = rest % P.n_ant;
+        const int pkt_l = rest / P.n_ant;
+        const int pkt = step * ppf + pkt_l;
+        const int chb = ch0 + 4 * grp;
+        const uint32_t wv[4] = {w4.x, w4.y, w4.z, w4.w};
+        _Float16 *row = lds16 + ant * RS;
+        if (gain_fast) {
+            // gains without a time mask: the slot's 8 complex gains loaded at once,
+            // then the arithmetic of cal_operand
+            float gr[4][2], gim[4][2];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    const int64_t gi_idx = ((int64_t)(chb + c) * 2 + p) * P.n_ant + ant;
+                    gr[c][p] = P.cal_re[gi_idx];
+                    gim[c][p] = P.cal_im[gi_idx];
+                }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t word = wv[c];
+                const int k = (pkt_l * P.csf_c + 4 * grp + c) * 2;
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    const uint32_t b0 = (word >> (8 * p)) & 0xFFu;
+                    const uint32_t b1 = (word >> (8 * (2 + p))) & 0xFFu;
+                    const float vr0 = (float)(_Float16)(nib_lo(b0) * P.scale);
+                    const float vi0 = (float)(_Float16)(nib_hi(b0) * P.scale);
+                    const float vr1 = (float)(_Float16)(nib_lo(b1) * P.scale);
+                    const float vi1 = (float)(_Float16)(nib_hi(b1) * P.scale);
+                    const float2 x0 = cal_mul(vr0, vi0, gr[c][p], gim[c][p]);
+                    const float2 x1 = cal_mul(vr1, vi1, gr[c][p], gim[c][p]);
+                    *reinterpret_cast<h2 *>(row + p * kc + k) = h2{(_Float16)x0.x, (_Float16)x1.x};
+                    *reinterpret_cast<h2 *>(row + 2 * kc + p * kc + k) = h2{(_Float16)x0.y, (_Float16)x1.y};
+                }
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t word = wv[c];                       // (t0 p0, t0 p1, t1 p0, t1 p1)
+                const int ch = chb + c;
+                const int k = (pkt_l * P.csf_c + 4 * grp + c) * 2;
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    const uint32_t b0 = (word >> (8 * p)) & 0xFFu;         // t0
+                    const uint32_t b1 = (word >> (8 * (2 + p))) & 0xFFu;   // t1
+                    const float vr0 = (float)(_Float16)(nib_lo(b0) * P.scale);
+                    const float vi0 = (float)(_Float16)(nib_hi(b0) * P.scale);
+                    const float vr1 = (float)(_Float16)(nib_lo(b1) * P.scale);
+                    const float vi1 = (float)(_Float16)(nib_hi(b1) * P.scale);
+                    _Float16 xr0, xi0, xr1, xi1;
+                    cal_operand(P, ch, p, ant, pkt, vr0, vi0, xr0, xi0);
+                    cal_operand(P, ch, p, ant, pkt, vr1, vi1, xr1, xi1);
+                    *reinterpret_cast<h2 *>(row + p * kc + k) = h2{xr0, xr1};
+                    *reinterpret_cast<h2 *>(row + 2 * kc + p * kc + k) = h2{xi0, xi1};
+                }
+            }
+        }
+    };
+    // prefetch: when a chunk's slots fit CCAL_PF per thread, the raw words of the
+    // next (item, chunk) are loaded into registers while this chunk's MFMAs run
+    const bool pf = P.vec4 && n_slot <= CCAL_PF * CCAL_THREADS;
+    uint4 wbuf[CCAL_PF];
+    auto fetch = [&](int it_f, int chunk_f) {
+        const int st_f = it_f / P.n_cout;
+        const int ch0_f = (it_f - st_f * P.n_cout) * P.csf + chunk_f * P.csf_c;
+#pragma unroll
+        for (int k = 0; k < CCAL_PF; ++k) {
+            const int sl = tid + k * CCAL_THREADS;
+            if (sl < n_slot) wbuf[k] = load_slot(sl, ch0_f, st_f);
+        }
+    };
+    if (pf && (int)blockIdx.x < P.n_items) fetch(blockIdx.x, 0);
+
     for (int item = blockIdx.x; item < P.n_items; item += gridDim.x) {
         const int step = item / P.n_cout;
         const int co = item - step * P.n_cout;
@@ -105,86 +200,14 @@ __global__ __launch_bounds__(CCAL_THREADS) void corr_cal_kernel(CalCorrParams P)
 
         for (int chunk = 0; chunk < P.n_chunks; ++chunk) {
             const int ch0 = co * P.csf + chunk * P.csf_c;
-            if (P.vec4) {
-                // one 16-byte load per (packet, antenna, 4-channel group); each channel's
-                // two steps stored as one fp16 pair per part
-                const int g4 = P.csf_c >> 2;
-                const int n_slot = ppf * P.n_ant * g4;
-                if (P.cal_re != nullptr && P.tkeep == nullptr) {
-                    // gains without a time mask: a slot's 8 complex gains and its raw
-                    // bytes are loaded together, then the arithmetic of cal_operand
-                    for (int sl = tid; sl < n_slot; sl += CCAL_THREADS) {
-                        const int grp = sl % g4;
-                        const int rest = sl / g4;
-                        const int ant = rest % P.n_ant;
-                        const int pkt_l = rest / P.n_ant;
-                        const int pkt = step * ppf + pkt_l;
-                        const int chb = ch0 + 4 * grp;
-                        float gr[4][2], gim[4][2];
+            if (pf) {
 #pragma unroll
-                        for (int c = 0; c < 4; ++c)
-#pragma unroll
-                            for (int p = 0; p < 2; ++p) {
-                                const int64_t gi_idx = ((int64_t)(chb + c) * 2 + p) * P.n_ant + ant;
-                                gr[c][p] = P.cal_re[gi_idx];
-                                gim[c][p] = P.cal_im[gi_idx];
-                            }
-                        const uint4 w4 = *reinterpret_cast<const uint4 *>(
-                            P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + chb) << 2));
-                        const uint32_t wv[4] = {w4.x, w4.y, w4.z, w4.w};
-                        _Float16 *row = lds16 + ant * RS;
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            const uint32_t word = wv[c];
-                            const int k = (pkt_l * P.csf_c + 4 * grp + c) * 2;
-#pragma unroll
-                            for (int p = 0; p < 2; ++p) {
-                                const uint32_t b0 = (word >> (8 * p)) & 0xFFu;
-                                const uint32_t b1 = (word >> (8 * (2 + p))) & 0xFFu;
-                                const float vr0 = (float)(_Float16)(nib_lo(b0) * P.scale);
-                                const float vi0 = (float)(_Float16)(nib_hi(b0) * P.scale);
-                                const float vr1 = (float)(_Float16)(nib_lo(b1) * P.scale);
-                                const float vi1 = (float)(_Float16)(nib_hi(b1) * P.scale);
-                                const float2 x0 = cal_mul(vr0, vi0, gr[c][p], gim[c][p]);
-                                const float2 x1 = cal_mul(vr1, vi1, gr[c][p], gim[c][p]);
-                                *reinterpret_cast<h2 *>(row + p * kc + k) = h2{(_Float16)x0.x, (_Float16)x1.x};
-                                *reinterpret_cast<h2 *>(row + 2 * kc + p * kc + k) = h2{(_Float16)x0.y, (_Float16)x1.y};
-                            }
-                        }
-                    }
-                } else
-                for (int sl = tid; sl < n_slot; sl += CCAL_THREADS) {
-                    const int grp = sl % g4;
-                    const int rest = sl / g4;
-                    const int ant = rest % P.n_ant;
-                    const int pkt_l = rest / P.n_ant;
-                    const int pkt = step * ppf + pkt_l;
-                    const int chb = ch0 + 4 * grp;
-                    const uint4 w4 = *reinterpret_cast<const uint4 *>(
-                        P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + chb) << 2));
-                    const uint32_t wv[4] = {w4.x, w4.y, w4.z, w4.w};
-                    _Float16 *row = lds16 + ant * RS;
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        const uint32_t word = wv[c];                       // (t0 p0, t0 p1, t1 p0, t1 p1)
-                        const int ch = chb + c;
-                        const int k = (pkt_l * P.csf_c + 4 * grp + c) * 2;
-#pragma unroll
-                        for (int p = 0; p < 2; ++p) {
-                            const uint32_t b0 = (word >> (8 * p)) & 0xFFu;         // t0
-                            const uint32_t b1 = (word >> (8 * (2 + p))) & 0xFFu;   // t1
-                            const float vr0 = (float)(_Float16)(nib_lo(b0) * P.scale);
-                            const float vi0 = (float)(_Float16)(nib_hi(b0) * P.scale);
-                            const float vr1 = (float)(_Float16)(nib_lo(b1) * P.scale);
-                            const float vi1 = (float)(_Float16)(nib_hi(b1) * P.scale);
-                            _Float16 xr0, xi0, xr1, xi1;
-                            cal_operand(P, ch, p, ant, pkt, vr0, vi0, xr0, xi0);
-                            cal_operand(P, ch, p, ant, pkt, vr1, vi1, xr1, xi1);
-                            *reinterpret_cast<h2 *>(row + p * kc + k) = h2{xr0, xr1};
-                            *reinterpret_cast<h2 *>(row + 2 * kc + p * kc + k) = h2{xi0, xi1};
-                        }
-                    }
+                for (int k = 0; k < CCAL_PF; ++k) {
+                    const int sl = tid + k * CCAL_THREADS;
+                    if (sl < n_slot) store_slot(wbuf[k], sl, ch0, step);
                 }
+            } else if (P.vec4) {
+                for (int sl = tid; sl < n_slot; sl += CCAL_THREADS) store_slot(load_slot(sl, ch0, step), sl, ch0, step);
             } else if (P.in_layout == 0) {
                 // one 4-byte word (t0 p0, t0 p1, t1 p0, t1 p1) per (packet, antenna, channel)
                 const int n_words = ppf * P.n_ant * P.csf_c;
@@ -237,6 +260,10 @@ __global__ __launch_bounds__(CCAL_THREADS) void corr_cal_kernel(CalCorrParams P)
                 }
             }
             __syncthreads();
+            if (pf) {
+                if (chunk + 1 < P.n_chunks) fetch(item, chunk + 1);
+                else if (item + (int)gridDim.x < P.n_items) fetch(item + gridDim.x, 0);
+            }
 
             // ---- MFMA over this wave's upper-triangle tile pairs (b tile <= a tile)
 #pragma unroll
