@@ -93,17 +93,18 @@ __global__ __launch_bounds__(CCAL_THREADS) void corr_cal_kernel(CalCorrParams P)
     for (int i = tid; i < n_vec; i += CCAL_THREADS) reinterpret_cast<v4i *>(lds16)[i] = v4i{0, 0, 0, 0};
     __syncthreads();
 
-    // vec4 input: slot sl = (packet, antenna, 4-channel group) of a chunk, one
-    // 16-byte load of raw bytes each
+    // vec4 input: slot sl = (packet, 4-channel group, antenna) of a chunk, antenna
+    // fastest so that a wave's gain loads read consecutive antennas; one 16-byte
+    // load of raw bytes each
     const int g4 = P.csf_c >> 2;
     const int n_slot = P.vec4 ? ppf * P.n_ant * g4 : 0;
     const bool gain_fast = P.cal_re != nullptr && P.tkeep == nullptr;
     auto load_slot = [&](int sl, int ch0, int step) -> uint4 {
         asm volatile("" : "+v"(sl));   // keep the slot's index math out of the item loop
-        const int grp = sl % g4;
-        const int rest = sl / g4;
-        const int ant = rest % P.n_ant;
-        const int pkt = step * ppf + rest / P.n_ant;
+        const int ant = sl % P.n_ant;
+        const int rest = sl / P.n_ant;
+        const int grp = rest % g4;
+        const int pkt = step * ppf + rest / g4;
         return *reinterpret_cast<const uint4 *>(
             P.raw + ((((int64_t)pkt * P.n_ant + ant) * P.n_chan + ch0 + 4 * grp) << 2));
     };
@@ -111,10 +112,10 @@ __global__ __launch_bounds__(CCAL_THREADS) void corr_cal_kernel(CalCorrParams P)
     // stored as one fp16 pair per part
     auto store_slot = [&](const uint4 w4, int sl, int ch0, int step) {
         asm volatile("" : "+v"(sl));
-        const int grp = sl % g4;
-        const int rest = sl / g4;
-        const int ant = rest % P.n_ant;
-        const int pkt_l = rest / P.n_ant;
+        const int ant = sl % P.n_ant;
+        const int rest = sl / P.n_ant;
+        const int grp = rest % g4;
+        const int pkt_l = rest / g4;
         const int pkt = step * ppf + pkt_l;
         const int chb = ch0 + 4 * grp;
         const uint32_t wv[4] = {w4.x, w4.y, w4.z, w4.w};
