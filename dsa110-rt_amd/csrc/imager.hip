@@ -360,14 +360,45 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
             // ---- 3. column C2R on MFMA, folded.  With theta = 2 pi r i / N,
             //         C[i][j] = sum_r c_r cos(theta) Re Y[r][j] and S[i][j] = -sum_r c_r sin(theta) Im Y[r][j]
             //         give out[i] = C + S and out[N - i] = C - S (cos is even and sin odd in i), so
-            //         only the output rows i in [0, N/2] are transformed: two MFMA chains per tile
+            //         only the output rows i in [0, N/2) are transformed: two MFMA chains per tile
             //         (A = cos or -sin, B = c_r Re Y or c_r Im Y, K = 4 uv rows per MFMA), and
             //         one LDS read of Y[r][j] feeds both
             constexpr int JT = IMG_JT;
             const int nkb = (rc + 3) / 4;
-            const int n_fold = (halfN + 16) / 16;            // i-tiles covering rows [0, N/2]
+            const int n_fold = (halfN + 15) / 16;            // i-tiles covering rows [0, N/2)
+            // ---- 4. epilogue of one pixel: scale, outer ring and DC pixel zeroed, cube
+            //         mask, chunk partial sums
+            auto emit = [&](int i, int j, float x) {
+                const bool inside = DENSE || (i >= P.npad && i < N - P.npad && j >= P.npad &&
+                                                j < N - P.npad && !(i == halfN && j == halfN));
+                const int64_t o = (int64_t)i * N + j;
+                float val = inside ? x * out_scale : 0.f;
+                if constexpr (DENSE == 2) {
+                    if (P.mask != nullptr) val *= P.mask[(int64_t)s * P.mask_stride + o];
+                }
+                if (DENSE == 2 && acc_out != nullptr) {
+                    // partial images in float32; the last pass writes the output type
+                    if (ck > 0) val += acc_out[o];
+                    if (ck + 1 < n_chunks) acc_out[o] = val;
+                    else st_from_float(out, o, val);
+                } else {
+                    if (ck > 0) val += ld_as_float(out, o);
+                    st_from_float(out, o, val);
+                }
+            };
+            // row N/2 on the VALU: theta = pi r there, so the sine terms vanish and
+            // out[N/2][j] = sum_r c_r (-1)^r Re Y[r][j]
+            for (int j = tid; j < N; j += IMG_THREADS) {
+                float x = 0.f;
+                for (int r = 0; r < rc; ++r) {
+                    const int rg = r0 + r;
+                    const float cr = (rg == 0 || rg == halfN) ? 1.f : 2.f;
+                    x += ((rg & 1) ? -cr : cr) * H[r * RS + j].x;
+                }
+                emit(halfN, j, x);
+            }
             // work units (i-tile, group of JT j-tiles) spread over the waves, so the
-            // N/32 + 1 folded i-tiles leave no wave idle
+            // N/32 folded i-tiles leave no wave idle
             const int n_jg = (n_tiles_1d + JT - 1) / JT;
             for (int unit = wave; unit < n_fold * n_jg; unit += IMG_THREADS / 64) {
                 const int it = unit / n_jg;
@@ -434,31 +465,12 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                     for (int h = 0; h < JT; ++h) {
                     if (jt + h >= n_tiles_1d) break;
                     const int j = (jt + h) * 16 + lr;
-                    // ---- 4. epilogue: scale, outer ring and DC pixel zeroed, cube mask,
-                    //         chunk partial sums; row i from C + S, row N - i from C - S
-                    auto emit = [&](int i, float x) {
-                        const bool inside = DENSE || (i >= P.npad && i < N - P.npad && j >= P.npad &&
-                                                        j < N - P.npad && !(i == halfN && j == halfN));
-                        const int64_t o = (int64_t)i * N + j;
-                        float val = inside ? x * out_scale : 0.f;
-                        if constexpr (DENSE == 2) {
-                            if (P.mask != nullptr) val *= P.mask[(int64_t)s * P.mask_stride + o];
-                        }
-                        if (DENSE == 2 && acc_out != nullptr) {
-                            // partial images in float32; the last pass writes the output type
-                            if (ck > 0) val += acc_out[o];
-                            if (ck + 1 < n_chunks) acc_out[o] = val;
-                            else st_from_float(out, o, val);
-                        } else {
-                            if (ck > 0) val += ld_as_float(out, o);
-                            st_from_float(out, o, val);
-                        }
-                    };
+                    // row i from C + S, row N - i from C - S
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int i = i0 + 4 * lg + v;
-                        if (i <= halfN) emit(i, acc_c[h][v] + acc_s[h][v]);
-                        if (i >= 1 && i < halfN) emit(N - i, acc_c[h][v] - acc_s[h][v]);
+                        if (i < halfN) emit(i, j, acc_c[h][v] + acc_s[h][v]);
+                        if (i >= 1 && i < halfN) emit(N - i, j, acc_c[h][v] - acc_s[h][v]);
                     }
                     }
                 }
