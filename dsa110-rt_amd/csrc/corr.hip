@@ -259,29 +259,36 @@ __global__ __launch_bounds__(CORR_THREADS) void corr_i8_kernel(CorrParams P) {
             v4i re[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
             v4i imp[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
             v4i imn[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+            // each polarisation in its own accumulators (a compile-time index: a
+            // runtime one makes every MFMA select its accumulator); fused Stokes I
+            // adds the two at the store, the same integer sums
+#pragma unroll
             for (int q = 0; q < 2; ++q) {
-                const int qo = P.fuse_pol ? 0 : q;
                 for (int kc = 0; kc < kq; kc += 64) {
                     const int off = q * kq + kc;
                     const v4i aR = *reinterpret_cast<const v4i *>(rowA + off);
                     const v4i aI = *reinterpret_cast<const v4i *>(rowA + 2 * kq + off);
                     const v4i bR = *reinterpret_cast<const v4i *>(rowB + off);
                     const v4i bI = *reinterpret_cast<const v4i *>(rowB + 2 * kq + off);
-                    re[qo] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aR, bR, re[qo], 0, 0, 0);
-                    re[qo] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aI, bI, re[qo], 0, 0, 0);
-                    imp[qo] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aR, bI, imp[qo], 0, 0, 0);
-                    imn[qo] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aI, bR, imn[qo], 0, 0, 0);
+                    re[q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aR, bR, re[q], 0, 0, 0);
+                    re[q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aI, bI, re[q], 0, 0, 0);
+                    imp[q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aR, bI, imp[q], 0, 0, 0);
+                    imn[q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aI, bR, imn[q], 0, 0, 0);
                 }
             }
             const int a = at * 16 + li;
-            const int npol_out = P.fuse_pol ? 1 : 2;
             if (a < P.n_ant) {
                 for (int v = 0; v < 4; ++v) {
                     const int b = bt * 16 + 4 * g + v;
                     if (b > a) continue;
                     const int bls = a * (a + 1) / 2 + b;
-                    for (int q = 0; q < npol_out; ++q)
-                        store_vis(P, step, co, q, bls, re[q][v], imp[q][v] - imn[q][v]);
+                    if (P.fuse_pol) {
+                        store_vis(P, step, co, 0, bls, re[0][v] + re[1][v],
+                                  (imp[0][v] + imp[1][v]) - (imn[0][v] + imn[1][v]));
+                    } else {
+                        store_vis(P, step, co, 0, bls, re[0][v], imp[0][v] - imn[0][v]);
+                        store_vis(P, step, co, 1, bls, re[1][v], imp[1][v] - imn[1][v]);
+                    }
                 }
             }
         }
