@@ -361,7 +361,7 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
             //         C[i][j] = sum_r c_r cos(theta) Re Y[r][j] and S[i][j] = -sum_r c_r sin(theta) Im Y[r][j]
             //         give out[i] = C + S and out[N - i] = C - S (cos is even and sin odd in i), so
             //         only the output rows i in [0, N/2) are transformed: two MFMA chains per tile
-            //         (A = cos or -sin, B = c_r Re Y or c_r Im Y, K = 4 uv rows per MFMA), and
+            //         (A = c_r cos or -c_r sin, B = Re Y or Im Y, K = 4 uv rows per MFMA), and
             //         one LDS read of Y[r][j] feeds both
             constexpr int JT = IMG_JT;
             const int nkb = (rc + 3) / 4;
@@ -412,9 +412,13 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                         const int r = kb * 4 + lg;
                         float c = 0.f, sn = 0.f;
                         if (kb < nkb && r < rc) {
-                            const float2 w = tw[((r0 + r) * i) & (N - 1)];
-                            c = w.x;
-                            sn = -w.y;
+                            // c_r (1 or 2) rides on the A operand: scaling by 2 is exact, so
+                            // the products equal those with c_r on Re Y / Im Y
+                            const int rg = r0 + r;
+                            const float cr = (rg == 0 || rg == halfN) ? 1.f : 2.f;
+                            const float2 w = tw[(rg * i) & (N - 1)];
+                            c = cr * w.x;
+                            sn = -(cr * w.y);
                         }
                         a_c[kb] = c;
                         a_s[kb] = sn;
@@ -440,14 +444,12 @@ __global__ __launch_bounds__(IMG_THREADS, 2) void image_kernel(ImgParams P) {
                                 const int jc = (jt + h < n_tiles_1d ? jt + h : jt) * 16;
 #pragma unroll
                                 for (int u = 0; u < IMG_KB_GROUP; ++u) {
+                                    // rows past the chunk read row 0 (finite) against a zero A
                                     const int r = (kb0 + u) * 4 + lg;
-                                    const bool ok = kb0 + u < nkb && r < rc;
-                                    const int rr = ok ? r : 0;
-                                    const int rg = r0 + rr;
-                                    const float cr = ok ? ((rg == 0 || rg == halfN) ? 1.f : 2.f) : 0.f;
+                                    const int rr = (kb0 + u < nkb && r < rc) ? r : 0;
                                     const float2 y = H[rr * RS + jc + lr];
-                                    bc[h][u] = cr * y.x;
-                                    bs[h][u] = cr * y.y;
+                                    bc[h][u] = y.x;
+                                    bs[h][u] = y.y;
                                 }
                             }
 #pragma unroll
